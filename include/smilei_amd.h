@@ -1,0 +1,201 @@
+/*
+ * smilei_amd.h -- C ABI of the MI355X-native PIC hot path.
+ *
+ * The library replaces Smilei's per-timestep particle loop
+ *   Species::dynamics -> Interpolator::fieldsWrapper -> Pusher::operator()
+ *                     -> Projector::currentsAndDensityWrapper
+ * plus the per-patch cell sort / exchange and the Yee field solver
+ * (MA_Solver3D_norm, MF_Solver3D_Yee, their 2D forms), with hand-written HIP
+ * kernels for gfx950.  Two layers are exported:
+ *
+ *  1. A context ("whole time step") API: particles and fields live in HBM,
+ *     smi_step() runs N complete PIC steps, batching every patch of every
+ *     species into single launches (optionally one captured hipGraph per
+ *     step).  This is the engine behind VectorPatch::dynamics /
+ *     sumDensities / solveMaxwell / finalizeExchParticlesAndSort
+ *     (src/Patch/VectorPatch.cpp:325, 819, 965; src/Patch/SyncVectorPatch.cpp:44).
+ *
+ *  2. Per-operator entry points on one patch in the reference's own memory
+ *     layout (ghosted per-patch Field3D/Field2D arrays, SoA particles,
+ *     SmileiMPI dynamics_* scratch), for a reference build that keeps its
+ *     own orchestration and swaps the operators only.
+ *
+ * All functions return SMI_OK (0) or a negative error code; smi_last_error()
+ * returns a text for the last failure of the calling thread.  Pointers named
+ * d_* are device pointers; all other array pointers are host pointers.
+ * Calls on one context must come from one host thread (the reference's GPU
+ * build uses one OpenMP thread per rank as well, src/Smilei.cpp:115-134).
+ */
+#ifndef SMILEI_AMD_H
+#define SMILEI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMI_OK 0
+#define SMI_ERR_INVALID (-1)  /* bad argument or unsupported namelist value */
+#define SMI_ERR_HIP (-2)      /* HIP runtime failure                        */
+#define SMI_ERR_STATE (-3)    /* call not valid in the current state         */
+#define SMI_ERR_CAPACITY (-4) /* a size exceeds what was allocated           */
+
+/* Precision forms of the particle path. */
+#define SMI_PLAIN_FP64 0 /* FP64 everywhere (the drop-in form)                 */
+#define SMI_MIXED 1      /* FP32 particle arithmetic, int32 cell + FP32 offset
+                            positions, FP64 fields and FP64 J reduction       */
+
+/* Current deposition modes (plain FP64 form). */
+#define SMI_DEPOSIT_ATOMIC 0        /* LDS tile per 4^3 cluster + FP64 atomics */
+#define SMI_DEPOSIT_DETERMINISTIC 1 /* per-patch ghosted J, particle order and
+                                       ghost-sum order of the reference        */
+
+/* Field identifiers (global periodic arrays [Nx][Ny][Nz], Nz=1 in 2D; a dual
+ * index g along an axis sits at (g - 1/2) * cell_length, like Smilei's). */
+#define SMI_EX 0
+#define SMI_EY 1
+#define SMI_EZ 2
+#define SMI_BX 3
+#define SMI_BY 4
+#define SMI_BZ 5
+#define SMI_BXM 6
+#define SMI_BYM 7
+#define SMI_BZM 8
+#define SMI_JX 9
+#define SMI_JY 10
+#define SMI_JZ 11
+#define SMI_NFIELDS 12
+
+typedef struct smi_context smi_context;
+
+/* Namelist subset consumed by the factories.  The string fields take the
+ * reference's namelist values and are checked the way its factories check
+ * them (InterpolatorFactory.h:44, PusherFactory.h:41, ProjectorFactory.h:32,
+ * SolverFactory.h:55,128); NULL selects the reference default. */
+typedef struct {
+    const char *geometry;       /* "2Dcartesian" | "3Dcartesian"           */
+    int interpolation_order;    /* 2 | 4 (4: 3Dcartesian only)             */
+    const char *interpolator;   /* "momentum-conserving" (default)         */
+    const char *pusher;         /* "boris" (default)                        */
+    const char *maxwell_solver; /* "Yee" (default)                          */
+    int n_cells[3];             /* global cells (Main.grid_length / cell_length) */
+    int patch_size[3];          /* cells per patch (multiple of 4)          */
+    double cell_length[3];
+    double timestep;
+    int precision;              /* SMI_PLAIN_FP64 | SMI_MIXED               */
+    int deposit;                /* SMI_DEPOSIT_ATOMIC | _DETERMINISTIC       */
+    int use_graph;              /* capture each time step in a hipGraph     */
+    int device;                 /* HIP device ordinal                      */
+} smi_params;
+
+const char *smi_last_error(void);
+
+/* ---- context API --------------------------------------------------- */
+int smi_create(const smi_params *params, smi_context **out);
+int smi_destroy(smi_context *ctx);
+/* mass in m_e, charge in e (integer-valued like Particles::charge);
+ * capacity = maximum number of macro-particles of the species. */
+int smi_add_species(smi_context *ctx, const char *name, double mass, double charge,
+                    int64_t capacity, int *species_id);
+/* Load particles (global coordinates, momenta u = gamma*beta in units of
+ * m_species*c, weights in the reference convention n*V_cell/ppc). */
+int smi_set_particles(smi_context *ctx, int species, int64_t n,
+                      const double *x, const double *y, const double *z,
+                      const double *px, const double *py, const double *pz,
+                      const double *w);
+/* Seeded in-HBM initialisation of a species: `ppc` particles per cell,
+ * uniform random in each cell, Gaussian momenta of std `sigma` (units of
+ * m_species*c) per component plus `drift` along x, weight `density*V/ppc`.
+ * Cells whose centre lies outside the cylinder of radius `radius` (in
+ * cells, around the (y,z) axis centre; <= 0 disables) get no particles. */
+int smi_init_species(smi_context *ctx, int species, int ppc, double density,
+                     double sigma, double drift_ux, double radius, uint64_t seed);
+int64_t smi_num_particles(const smi_context *ctx, int species);
+/* Particles in canonical order (patch-major, then cell key, then arrival
+ * order); patch and key receive the owning patch id ((px*npy+py)*npz+pz)
+ * and the in-patch cell key.  Any pointer may be NULL. */
+int smi_get_particles(smi_context *ctx, int species, double *x, double *y, double *z,
+                      double *px, double *py, double *pz, double *w,
+                      int32_t *patch, int32_t *key);
+int smi_patch_counts(smi_context *ctx, int species, int32_t *counts);
+int smi_set_field(smi_context *ctx, int field, const double *global);
+int smi_get_field(smi_context *ctx, int field, double *global);
+/* Run n complete time steps (stream-ordered; returns after enqueueing). */
+int smi_step(smi_context *ctx, int nsteps);
+int smi_synchronize(smi_context *ctx);
+/* out[0] = Uelm, out[1+s] = Ukin of species s (DiagnosticScalar formulas). */
+int smi_energies(smi_context *ctx, double *out);
+/* Charge density (all species) and div(E) - rho on primal nodes. */
+int smi_rho(smi_context *ctx, double *global);
+int smi_gauss(smi_context *ctx, double *global);
+/* Per-kernel-class device time of one step, measured with HIP events on the
+ * context's stream: ms[0] particle kernel (gather+push+deposit), ms[1] sort,
+ * ms[2] J reduction (deterministic mode), ms[3] Maxwell. */
+int smi_profile_step(smi_context *ctx, float *ms);
+/* The stream all work of the context is enqueued on (hipStream_t). */
+void *smi_stream(smi_context *ctx);
+
+/* ---- per-operator entry points (one patch, reference layout) -------- *
+ * Field arrays are the patch's ghosted arrays: primal extent
+ * np[d] = patch_size[d] + 2*oversize + 1, +1 along each staggered axis
+ * (ElectroMagn3D.cpp:88-180); dom_begin = cell_starting_global_index
+ * (Patch.cpp:145-162).  Scratch arrays have stride n as SmileiMPI's
+ * dynamics_Epart/Bpart/iold/deltaold/invgf (SmileiMPI.h:213-221).      */
+
+/* Interpolator3D2Order / Interpolator3D4Order ::fieldsWrapper
+ * (src/Interpolator/Interpolator3D2Order.cpp:163-285,
+ *  src/Interpolator/Interpolator3D4Order.cpp:159-). */
+int smi_interpolate_3d(int order, const double *d_Ex, const double *d_Ey, const double *d_Ez,
+                       const double *d_Bxm, const double *d_Bym, const double *d_Bzm,
+                       const int np[3], const int dom_begin[3], const double cell_length[3],
+                       int64_t n, const double *d_x, const double *d_y, const double *d_z,
+                       double *d_Epart, double *d_Bpart, int32_t *d_iold, double *d_deltaold,
+                       void *stream);
+/* Interpolator2D2Order::fieldsWrapper (src/Interpolator/Interpolator2D2Order.cpp:158-). */
+int smi_interpolate_2d(int order, const double *d_Ex, const double *d_Ey, const double *d_Ez,
+                       const double *d_Bxm, const double *d_Bym, const double *d_Bzm,
+                       const int np[2], const int dom_begin[2], const double cell_length[2],
+                       int64_t n, const double *d_x, const double *d_y,
+                       double *d_Epart, double *d_Bpart, int32_t *d_iold, double *d_deltaold,
+                       void *stream);
+/* PusherBoris::operator() (src/Pusher/PusherBoris.cpp:23-142). */
+int smi_push_boris(int ndim, double charge, double mass, double timestep, int64_t n,
+                   double *d_x, double *d_y, double *d_z,
+                   double *d_px, double *d_py, double *d_pz,
+                   const double *d_Epart, const double *d_Bpart, double *d_invgf, void *stream);
+/* Projector3D2Order / Projector3D4Order ::currents over particles [0,n)
+ * (src/Projector/Projector3D2Order.cpp:55-343, 733-766;
+ *  src/Projector/Projector3D4Order.cpp:49-237, 682-716); replaces the
+ * reference's extern "C" currentDeposition3DOnDevice
+ * (src/Projector/Projector3D2OrderGPU.cpp:55-88).  deterministic != 0 adds
+ * the particles in index order (bit-identical to the CPU operator),
+ * otherwise FP64 atomics. */
+int smi_currents_3d(int order, double *d_Jx, double *d_Jy, double *d_Jz,
+                    const int np[3], const int dom_begin[3], const double cell_length[3],
+                    double timestep, double charge, int64_t n,
+                    const double *d_x, const double *d_y, const double *d_z, const double *d_w,
+                    const int32_t *d_iold, const double *d_deltaold, int deterministic,
+                    void *stream);
+/* Projector2D2Order::currents (src/Projector/Projector2D2Order.cpp:49-189). */
+int smi_currents_2d(int order, double *d_Jx, double *d_Jy, double *d_Jz,
+                    const int np[2], const int dom_begin[2], const double cell_length[2],
+                    double timestep, double charge, int64_t n,
+                    const double *d_x, const double *d_y, const double *d_pz, const double *d_w,
+                    const double *d_invgf, const int32_t *d_iold, const double *d_deltaold,
+                    int deterministic, void *stream);
+/* MA_Solver3D_norm / MF_Solver3D_Yee ::operator()
+ * (src/ElectroMagnSolver/MA_Solver3D_norm.cpp:18-115,
+ *  src/ElectroMagnSolver/MF_Solver3D_Yee.cpp:17-110). */
+int smi_maxwell_ampere_3d(double *d_Ex, double *d_Ey, double *d_Ez,
+                          const double *d_Bx, const double *d_By, const double *d_Bz,
+                          const double *d_Jx, const double *d_Jy, const double *d_Jz,
+                          const int np[3], double timestep, const double cell_length[3], void *stream);
+int smi_maxwell_faraday_3d(double *d_Bx, double *d_By, double *d_Bz,
+                           const double *d_Ex, const double *d_Ey, const double *d_Ez,
+                           const int np[3], double timestep, const double cell_length[3], void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,142 @@
+"""smilei_amd -- MI355X-native PIC time step of Smilei behind a C ABI.
+
+Python is only the driver: the particle loop, the sort/exchange and the Yee
+solver run in libsmilei_amd.so (HIP, gfx950).  ``Simulation`` wraps one
+context of the C ABI (include/smilei_amd.h); ``namelist`` builds one from
+Smilei-style ``Main`` / ``Species`` blocks.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SmiParams, check
+
+__all__ = ["Simulation", "FIELDS", "PLAIN_FP64", "MIXED", "DEPOSIT_ATOMIC", "DEPOSIT_DETERMINISTIC"]
+
+PLAIN_FP64, MIXED = 0, 1
+DEPOSIT_ATOMIC, DEPOSIT_DETERMINISTIC = 0, 1
+FIELDS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"]
+_FID = {n: i for i, n in enumerate(FIELDS)}
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+class Simulation:
+    """One simulation (all patches, all species) resident on one GPU.
+
+    Parameters follow Smilei's namelist: ``geometry`` ("2Dcartesian" /
+    "3Dcartesian"), ``interpolation_order`` (2, or 4 in 3D), ``n_cells``
+    (global cells per axis), ``patch_size`` (cells per patch, multiple of 4),
+    ``cell_length``, ``timestep``; ``deposit`` selects the FP64 atomic tile
+    deposit or the deterministic per-patch one (reference summation order).
+    """
+
+    def __init__(self, geometry="3Dcartesian", interpolation_order=2, n_cells=(32, 32, 32),
+                 patch_size=(16, 16, 16), cell_length=(0.2, 0.2, 0.2), timestep=0.1,
+                 precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True, device=0,
+                 interpolator="momentum-conserving", pusher="boris", maxwell_solver="Yee"):
+        self._L = _lib.load()
+        nd = 2 if geometry == "2Dcartesian" else 3
+        p = SmiParams()
+        self._keep = [s.encode() for s in (geometry, interpolator, pusher, maxwell_solver)]
+        p.geometry, p.interpolator, p.pusher, p.maxwell_solver = self._keep
+        p.interpolation_order = int(interpolation_order)
+        nc = list(n_cells) + [1] * (3 - len(n_cells))
+        ps = list(patch_size) + [1] * (3 - len(patch_size))
+        cl = list(cell_length) + [1.0] * (3 - len(cell_length))
+        for d in range(3):
+            p.n_cells[d], p.patch_size[d], p.cell_length[d] = int(nc[d]), int(ps[d]), float(cl[d])
+        p.timestep = float(timestep)
+        p.precision, p.deposit, p.use_graph, p.device = int(precision), int(deposit), int(bool(use_graph)), int(device)
+        h = ctypes.c_void_p()
+        check(self._L.smi_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.ndim = nd
+        self.shape = tuple(int(nc[d]) if d < nd else 1 for d in range(3))
+        self.npatch = int(np.prod([self.shape[d] // int(ps[d]) if d < nd else 1 for d in range(3)]))
+        self.species = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.smi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- species / particles
+    def add_species(self, name, mass, charge, capacity):
+        sid = ctypes.c_int()
+        check(self._L.smi_add_species(self.h, name.encode(), float(mass), float(charge), int(capacity),
+                                      ctypes.byref(sid)))
+        self.species.append(name)
+        return sid.value
+
+    def set_particles(self, s, x, y, z, px, py, pz, w):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, px, py, pz, w)]
+        n = len(arrs[-1])
+        check(self._L.smi_set_particles(self.h, s, n, *[_dp(a) for a in arrs]))
+
+    def init_species(self, s, ppc, density, sigma, drift_ux=0.0, radius=0.0, seed=0):
+        check(self._L.smi_init_species(self.h, s, int(ppc), float(density), float(sigma), float(drift_ux),
+                                       float(radius), int(seed)))
+
+    def num_particles(self, s):
+        return self._L.smi_num_particles(self.h, s)
+
+    def get_particles(self, s):
+        n = self.num_particles(s)
+        out = {k: np.empty(n) for k in ("x", "y", "z", "px", "py", "pz", "w")}
+        out["patch"] = np.empty(n, np.int32)
+        out["key"] = np.empty(n, np.int32)
+        check(self._L.smi_get_particles(self.h, s, *[_dp(out[k]) for k in ("x", "y", "z", "px", "py", "pz", "w")],
+                                        out["patch"].ctypes.data_as(_I), out["key"].ctypes.data_as(_I)))
+        return out
+
+    def patch_counts(self, s):
+        c = np.empty(self.npatch, np.int32)
+        check(self._L.smi_patch_counts(self.h, s, c.ctypes.data_as(_I)))
+        return c
+
+    # ---- fields
+    def set_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
+        check(self._L.smi_set_field(self.h, _FID[name], _dp(a)))
+
+    def get_field(self, name):
+        a = np.empty(self.shape)
+        check(self._L.smi_get_field(self.h, _FID[name], _dp(a)))
+        return a
+
+    # ---- time loop
+    def step(self, n=1):
+        check(self._L.smi_step(self.h, int(n)))
+
+    def synchronize(self):
+        check(self._L.smi_synchronize(self.h))
+
+    def profile_step(self):
+        ms = (ctypes.c_float * 4)()
+        check(self._L.smi_profile_step(self.h, ms))
+        return {"particles": ms[0], "sort": ms[1], "j_reduce": ms[2], "maxwell": ms[3]}
+
+    # ---- diagnostics
+    def energies(self):
+        out = np.empty(1 + len(self.species))
+        check(self._L.smi_energies(self.h, _dp(out)))
+        return out
+
+    def rho(self):
+        a = np.empty(self.shape)
+        check(self._L.smi_rho(self.h, _dp(a)))
+        return a
+
+    def gauss(self):
+        a = np.empty(self.shape)
+        check(self._L.smi_gauss(self.h, _dp(a)))
+        return a

@@ -1,0 +1,623 @@
+// engine.cpp -- context API of the MI355X PIC library (host side).
+//
+// A context owns one simulation on one GPU: the periodic global field
+// arrays, the particles of every species in HBM (double-buffered SoA plus
+// the sorted view `perm`), the per-patch scratch of the deterministic
+// deposit, and one HIP stream.  smi_step() enqueues, per time step, the
+// sequence of VectorPatch::dynamics (src/Patch/VectorPatch.cpp:4763),
+// sumDensities (:819), solveMaxwell (:965) and finalizeExchParticlesAndSort
+// (src/Patch/SyncVectorPatch.cpp:44) as a handful of batched launches, and
+// can replay it as a captured hipGraph.
+//
+// The namelist checks mirror the reference factories: unsupported values
+// are rejected with SMI_ERR_INVALID and a message instead of ERROR()+abort.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/smilei_amd.h"
+#include "kernels.h"
+
+using namespace smi;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr int kReduceBlocks = 1024;
+
+struct SpeciesState {
+    std::string name;
+    double mass = 1, charge = -1;
+    int64_t cap = 0, n = 0;
+    double *buf[2][7] = {};
+    uint32_t *keys = nullptr;
+    SortBufs sb{};
+    double *scratch = nullptr;  // 7*cap, deterministic mode
+    SpeciesK k{};
+
+    PartPtrs view(int b) const
+    {
+        return PartPtrs{buf[b][0], buf[b][1], buf[b][2], buf[b][3], buf[b][4], buf[b][5], buf[b][6]};
+    }
+};
+
+template <class T>
+T *dalloc(int64_t n)
+{
+    void *p = nullptr;
+    if (n <= 0) n = 1;
+    SMI_HIP(hipMalloc(&p, (size_t)n * sizeof(T)));
+    return (T *)p;
+}
+
+bool streq(const char *a, const char *b) { return a && b && std::strcmp(a, b) == 0; }
+
+}  // namespace
+
+struct smi_context {
+    smi_params p{};
+    Geom g{};
+    hipStream_t st = nullptr;
+    double *F[SMI_NFIELDS] = {};
+    std::vector<SpeciesState> sp;
+    double *pj = nullptr;          // per-patch ghosted J (deterministic mode)
+    DetSpecies *d_det = nullptr;   // device copy of the deposit descriptors
+    int d_det_n = 0;
+    double *partials = nullptr;    // reduction scratch
+    double *rho = nullptr;         // diagnostics scratch (ncells)
+    int *err = nullptr;            // device flag raised by kernels on corrupt particles
+    int cur = 0;                   // particle buffer holding the current state
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipEvent_t ev[6] = {};
+
+    FieldSet fields() const
+    {
+        FieldSet f;
+        for (int c = 0; c < 3; ++c) {
+            f.E[c] = F[SMI_EX + c];
+            f.B[c] = F[SMI_BX + c];
+            f.Bm[c] = F[SMI_BXM + c];
+            f.J[c] = F[SMI_JX + c];
+        }
+        return f;
+    }
+};
+
+namespace {
+
+void check_ctx(const smi_context *c)
+{
+    if (!c) throw Error(SMI_ERR_INVALID, "null context");
+}
+
+void drop_graphs(smi_context *c)
+{
+    for (auto &gx : c->graph)
+        if (gx) {
+            (void)hipGraphExecDestroy(gx);
+            gx = nullptr;
+        }
+}
+
+// Device descriptors of the deterministic deposit for both ping-pong states
+// (set k: input buffer k, output buffer 1-k), refreshed when species change.
+void upload_det(smi_context *c)
+{
+    if (c->sp.empty()) return;
+    std::vector<DetSpecies> v;
+    for (int k = 0; k < 2; ++k)
+        for (auto &s : c->sp) {
+            DetSpecies d;
+            d.sp = s.k;
+            d.cell_off = s.sb.cell_off;
+            d.scratch = s.scratch;
+            d.pz = s.buf[1 - k][5];
+            d.w = s.buf[1 - k][6];
+            d.invgf = nullptr;
+            d.cap = s.cap;
+            v.push_back(d);
+        }
+    SMI_HIP(hipMemcpyAsync(c->d_det, v.data(), v.size() * sizeof(DetSpecies), hipMemcpyHostToDevice, c->st));
+}
+
+// One PIC time step reading particle buffer c->cur and writing 1 - c->cur.
+void enqueue_step(smi_context *c, bool timed)
+{
+    const int a = c->cur;
+    const Geom &g = c->g;
+    const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
+    FieldSet f = c->fields();
+    JPtrs J{f.J[0], f.J[1], f.J[2]};
+    GatherFields GF{f.E[0], f.E[1], f.E[2], f.Bm[0], f.Bm[1], f.Bm[2]};
+    if (timed) SMI_HIP(hipEventRecord(c->ev[0], c->st));
+    if (!det)
+        for (int q = 0; q < 3; ++q) launch_fill(f.J[q], g.ncells, 0.0, c->st);  // restartRhoJ
+    for (auto &s : c->sp) {
+        launch_particles(g, s.k, s.view(a), s.sb.perm, s.view(1 - a), s.keys, s.scratch, s.cap, s.sb.cell_off, GF, J,
+                         !det, c->err, c->st);
+    }
+    if (timed) SMI_HIP(hipEventRecord(c->ev[1], c->st));
+    if (det && !c->sp.empty()) {
+        launch_det_deposit(g, c->d_det + a * c->d_det_n, (int)c->sp.size(), c->pj, c->err, c->st);
+        launch_ghost_sum_and_fold(g, c->pj, J, c->st);
+    }
+    if (timed) SMI_HIP(hipEventRecord(c->ev[2], c->st));
+    launch_maxwell(g, f, c->st);
+    if (timed) SMI_HIP(hipEventRecord(c->ev[3], c->st));
+    for (auto &s : c->sp) launch_sort(g, s.keys, s.n, s.sb, c->st);
+    if (timed) SMI_HIP(hipEventRecord(c->ev[4], c->st));
+}
+
+void flip(smi_context *c) { c->cur ^= 1; }
+
+int fail(const Error &e)
+{
+    g_last_error = e.what();
+    return e.code;
+}
+
+}  // namespace
+
+void smi::set_error(const std::string &msg) { g_last_error = msg; }
+
+#define SMI_TRY try {
+#define SMI_CATCH                                                    \
+    }                                                                \
+    catch (const Error &e) { return fail(e); }                       \
+    catch (const std::exception &e) { return fail(Error(SMI_ERR_INVALID, e.what())); }
+
+extern "C" {
+
+const char *smi_last_error(void) { return g_last_error.c_str(); }
+
+int smi_create(const smi_params *p, smi_context **out)
+{
+    SMI_TRY
+    if (!p || !out) throw Error(SMI_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int nd;
+    if (streq(p->geometry, "3Dcartesian") || !p->geometry) nd = 3;
+    else if (streq(p->geometry, "2Dcartesian")) nd = 2;
+    else throw Error(SMI_ERR_INVALID, std::string("unsupported geometry: ") + p->geometry);
+    // InterpolatorFactory / ProjectorFactory: order 2 (2D, 3D) or 4 (3D)
+    if (!(p->interpolation_order == 2 || (p->interpolation_order == 4 && nd == 3)))
+        throw Error(SMI_ERR_INVALID, "interpolation_order must be 2 (2D/3D) or 4 (3D)");
+    if (p->interpolator && !streq(p->interpolator, "momentum-conserving"))
+        throw Error(SMI_ERR_INVALID, std::string("unsupported interpolator: ") + p->interpolator);
+    if (p->pusher && !streq(p->pusher, "boris") && !streq(p->pusher, "Boris"))
+        throw Error(SMI_ERR_INVALID, std::string("unsupported pusher: ") + p->pusher);
+    if (p->maxwell_solver && !streq(p->maxwell_solver, "Yee"))
+        throw Error(SMI_ERR_INVALID, std::string("unsupported maxwell_solver: ") + p->maxwell_solver);
+    if (p->precision != SMI_PLAIN_FP64 && p->precision != SMI_MIXED)
+        throw Error(SMI_ERR_INVALID, "precision must be SMI_PLAIN_FP64 or SMI_MIXED");
+    if (p->precision == SMI_MIXED) throw Error(SMI_ERR_INVALID, "mixed precision form not available in this build");
+    if (p->deposit != SMI_DEPOSIT_ATOMIC && p->deposit != SMI_DEPOSIT_DETERMINISTIC)
+        throw Error(SMI_ERR_INVALID, "deposit must be SMI_DEPOSIT_ATOMIC or SMI_DEPOSIT_DETERMINISTIC");
+    if (!(p->timestep > 0)) throw Error(SMI_ERR_INVALID, "timestep must be > 0");
+
+    Geom g{};
+    g.nd = nd;
+    g.order = p->interpolation_order;
+    g.os = g.order;
+    int64_t ncells = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (d < nd) {
+            if (p->n_cells[d] <= 0 || p->patch_size[d] <= 0 || !(p->cell_length[d] > 0))
+                throw Error(SMI_ERR_INVALID, "n_cells, patch_size and cell_length must be positive");
+            if (p->patch_size[d] % 4) throw Error(SMI_ERR_INVALID, "patch_size must be a multiple of 4");
+            if (p->n_cells[d] % p->patch_size[d])
+                throw Error(SMI_ERR_INVALID, "patch_size must divide n_cells (Params.cpp:1210)");
+            if (p->patch_size[d] < 2 * g.os + 2)
+                throw Error(SMI_ERR_INVALID, "patch_size must be >= 2*oversize+2 for the ghost sum");
+            g.N[d] = p->n_cells[d];
+            g.n[d] = p->patch_size[d];
+            g.dx[d] = p->cell_length[d];
+        } else {
+            g.N[d] = 1;
+            g.n[d] = 1;
+            g.dx[d] = 1.0;
+        }
+        g.npatch[d] = g.N[d] / g.n[d];
+        g.ncl[d] = d < nd ? g.n[d] / 4 : 1;
+        g.dinv[d] = 1.0 / g.dx[d];
+        g.plen[d] = (double)g.n[d] * g.dx[d];
+        g.xmax[d] = g.dx[d] * (double)g.N[d];
+        ncells *= g.N[d];
+    }
+    if (ncells >= (1ll << 27)) throw Error(SMI_ERR_INVALID, "at most 2^27 cells per simulation");
+    g.ncells = ncells;
+    g.cpp = g.n[0] * g.n[1] * g.n[2];
+    g.clpp = g.ncl[0] * g.ncl[1] * g.ncl[2];
+    g.cells_per_cl = nd == 3 ? 64 : 16;
+    g.npatches = g.npatch[0] * g.npatch[1] * g.npatch[2];
+    g.dt = p->timestep;
+
+    SMI_HIP(hipSetDevice(p->device));
+    auto *c = new smi_context();
+    c->p = *p;
+    c->g = g;
+    try {
+        SMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        for (int f = 0; f < SMI_NFIELDS; ++f) {
+            c->F[f] = dalloc<double>(ncells);
+            launch_fill(c->F[f], ncells, 0.0, c->st);
+        }
+        c->partials = dalloc<double>(kReduceBlocks);
+        c->rho = dalloc<double>(ncells);
+        c->err = dalloc<int>(1);
+        launch_fill_i32(c->err, 1, 0, c->st);
+        if (p->deposit == SMI_DEPOSIT_DETERMINISTIC) c->pj = dalloc<double>(patch_j_size(g) * g.npatches);
+        for (auto &e : c->ev) SMI_HIP(hipEventCreate(&e));
+        SMI_HIP(hipStreamSynchronize(c->st));
+    } catch (...) {
+        smi_destroy(c);
+        throw;
+    }
+    *out = c;
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_destroy(smi_context *c)
+{
+    if (!c) return SMI_OK;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    drop_graphs(c);
+    for (auto &f : c->F) (void)hipFree(f);
+    for (auto &s : c->sp) {
+        for (auto &bb : s.buf)
+            for (auto &q : bb) (void)hipFree(q);
+        (void)hipFree(s.keys);
+        (void)hipFree(s.sb.counts);
+        (void)hipFree(s.sb.cell_off);
+        (void)hipFree(s.sb.perm);
+        (void)hipFree(s.sb.bsum);
+        (void)hipFree(s.scratch);
+    }
+    (void)hipFree(c->pj);
+    (void)hipFree(c->d_det);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->rho);
+    (void)hipFree(c->err);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return SMI_OK;
+}
+
+int smi_add_species(smi_context *c, const char *name, double mass, double charge, int64_t capacity, int *id)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (!(mass > 0)) throw Error(SMI_ERR_INVALID, "mass must be > 0");
+    if (capacity < 0 || capacity >= (1ll << 31)) throw Error(SMI_ERR_INVALID, "capacity must be in [0, 2^31)");
+    SMI_HIP(hipSetDevice(c->p.device));
+    drop_graphs(c);
+    SpeciesState s;
+    s.name = name ? name : "";
+    s.mass = mass;
+    s.charge = charge;
+    s.cap = capacity;
+    for (auto &bb : s.buf)
+        for (auto &q : bb) q = dalloc<double>(capacity);
+    s.keys = dalloc<uint32_t>(capacity);
+    s.sb.counts = dalloc<int32_t>(c->g.ncells);
+    s.sb.cell_off = dalloc<int32_t>(c->g.ncells + 1);
+    s.sb.perm = dalloc<int32_t>(capacity);
+    s.sb.nbsum = sort_bsum_size(c->g.ncells);
+    s.sb.bsum = dalloc<int32_t>(s.sb.nbsum);
+    if (c->p.deposit == SMI_DEPOSIT_DETERMINISTIC) s.scratch = dalloc<double>(7 * capacity);
+    launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
+    launch_fill_i32(s.sb.cell_off, c->g.ncells + 1, 0, c->st);
+    double cell_volume = 1.0;
+    for (int d = 0; d < c->g.nd; ++d) cell_volume *= c->g.dx[d];
+    const double one_over_mass = 1.0 / mass;
+    const double dts2 = c->g.dt / 2.;
+    s.k.charge = charge;
+    s.k.mass = mass;
+    s.k.qmh = charge * one_over_mass * dts2;
+    s.k.cw_scale = (1. / cell_volume) * charge;
+    c->sp.push_back(s);
+    // deterministic-deposit descriptors for both ping-pong parities
+    (void)hipFree(c->d_det);
+    c->d_det = nullptr;
+    c->d_det_n = (int)c->sp.size();
+    c->d_det = dalloc<DetSpecies>(2 * c->d_det_n);
+    upload_det(c);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    if (id) *id = (int)c->sp.size() - 1;
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const double *y, const double *z,
+                      const double *px, const double *py, const double *pz, const double *w)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
+    SpeciesState &S = c->sp[s];
+    if (n < 0 || n > S.cap) throw Error(SMI_ERR_CAPACITY, "particle count exceeds the species capacity");
+    const double *src[7] = {x, y, z, px, py, pz, w};
+    for (int d = 0; d < c->g.nd; ++d) {
+        const double *v = src[d];
+        for (int64_t i = 0; i < n; ++i)
+            if (!(v[i] >= 0. && v[i] < c->g.xmax[d]))
+                throw Error(SMI_ERR_INVALID, "particle position outside the simulation box");
+    }
+    SMI_HIP(hipSetDevice(c->p.device));
+    drop_graphs(c);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    for (int q = 0; q < 7; ++q) {
+        if (q == 2 && c->g.nd < 3) {
+            launch_fill(S.buf[c->cur][q], n, 0.0, c->st);
+            continue;
+        }
+        if (!src[q]) throw Error(SMI_ERR_INVALID, "null particle array");
+        if (n) SMI_HIP(hipMemcpyAsync(S.buf[c->cur][q], src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    }
+    S.n = n;
+    launch_bin_initial(c->g, S.view(c->cur), n, S.keys, c->st);
+    launch_sort(c->g, S.keys, n, S.sb, c->st);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_init_species(smi_context *c, int s, int ppc, double density, double sigma, double drift_ux, double radius,
+                     uint64_t seed)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
+    if (ppc <= 0) throw Error(SMI_ERR_INVALID, "ppc must be > 0");
+    SpeciesState &S = c->sp[s];
+    const Geom &g = c->g;
+    SMI_HIP(hipSetDevice(c->p.device));
+    drop_graphs(c);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    int32_t *d_cells = nullptr;
+    int64_t ncell_act = g.ncells;
+    if (radius > 0) {
+        // cells whose centre lies within `radius` cells of the transverse axis
+        std::vector<int32_t> act;
+        const double cy = 0.5 * g.N[1], cz = 0.5 * g.N[2];
+        for (int64_t t = 0; t < g.ncells; ++t) {
+            const int k = (int)(t % g.N[2]), j = (int)((t / g.N[2]) % g.N[1]);
+            const double ry = j + 0.5 - cy, rz = g.nd == 3 ? k + 0.5 - cz : 0.;
+            if (ry * ry + rz * rz <= radius * radius) act.push_back((int32_t)t);
+        }
+        ncell_act = (int64_t)act.size();
+        d_cells = dalloc<int32_t>(ncell_act);
+        if (ncell_act)
+            SMI_HIP(hipMemcpy(d_cells, act.data(), act.size() * 4, hipMemcpyHostToDevice));
+    }
+    const int64_t n = ncell_act * ppc;
+    if (n > S.cap) {
+        (void)hipFree(d_cells);
+        throw Error(SMI_ERR_CAPACITY, "initial particles exceed the species capacity");
+    }
+    double cell_volume = 1.0;
+    for (int d = 0; d < g.nd; ++d) cell_volume *= g.dx[d];
+    launch_init(g, S.view(c->cur), n, ppc, d_cells, density * cell_volume / ppc, sigma, drift_ux,
+                seed * 0x100000001B3ull + (uint64_t)s, c->st);
+    S.n = n;
+    launch_bin_initial(g, S.view(c->cur), n, S.keys, c->st);
+    launch_sort(g, S.keys, n, S.sb, c->st);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    (void)hipFree(d_cells);
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int64_t smi_num_particles(const smi_context *c, int s)
+{
+    if (!c || s < 0 || s >= (int)c->sp.size()) return -1;
+    return c->sp[s].n;
+}
+
+int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, double *px, double *py, double *pz,
+                      double *w, int32_t *patch, int32_t *key)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
+    SpeciesState &S = c->sp[s];
+    const int64_t n = S.n;
+    SMI_HIP(hipSetDevice(c->p.device));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    std::vector<int32_t> perm(n);
+    std::vector<uint32_t> keys(n);
+    std::vector<double> tmp(n);
+    if (n) {
+        SMI_HIP(hipMemcpy(perm.data(), S.sb.perm, n * 4, hipMemcpyDeviceToHost));
+        SMI_HIP(hipMemcpy(keys.data(), S.keys, n * 4, hipMemcpyDeviceToHost));
+    }
+    double *dst[7] = {x, y, z, px, py, pz, w};
+    for (int q = 0; q < 7; ++q) {
+        if (!dst[q]) continue;
+        if (q == 2 && c->g.nd < 3) {
+            std::fill(dst[q], dst[q] + n, 0.0);
+            continue;
+        }
+        if (n) SMI_HIP(hipMemcpy(tmp.data(), S.buf[c->cur][q], n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < n; ++j) dst[q][j] = tmp[perm[j]];
+    }
+    const uint32_t mask = (1u << 27) - 1;
+    for (int64_t j = 0; j < n; ++j) {
+        const uint32_t gk = keys[perm[j]] & mask;
+        if (patch) patch[j] = (int32_t)(gk / (uint32_t)c->g.cpp);
+        if (key) key[j] = (int32_t)(gk % (uint32_t)c->g.cpp);
+    }
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_patch_counts(smi_context *c, int s, int32_t *counts)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
+    SMI_HIP(hipSetDevice(c->p.device));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    std::vector<int32_t> off(c->g.ncells + 1);
+    SMI_HIP(hipMemcpy(off.data(), c->sp[s].sb.cell_off, off.size() * 4, hipMemcpyDeviceToHost));
+    for (int p = 0; p < c->g.npatches; ++p)
+        counts[p] = off[(int64_t)(p + 1) * c->g.cpp] - off[(int64_t)p * c->g.cpp];
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_set_field(smi_context *c, int f, const double *gl)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (f < 0 || f >= SMI_NFIELDS || !gl) throw Error(SMI_ERR_INVALID, "bad field id");
+    SMI_HIP(hipSetDevice(c->p.device));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    SMI_HIP(hipMemcpy(c->F[f], gl, c->g.ncells * sizeof(double), hipMemcpyHostToDevice));
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_get_field(smi_context *c, int f, double *gl)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (f < 0 || f >= SMI_NFIELDS || !gl) throw Error(SMI_ERR_INVALID, "bad field id");
+    SMI_HIP(hipSetDevice(c->p.device));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    SMI_HIP(hipMemcpy(gl, c->F[f], c->g.ncells * sizeof(double), hipMemcpyDeviceToHost));
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_step(smi_context *c, int nsteps)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (nsteps < 0) throw Error(SMI_ERR_INVALID, "nsteps < 0");
+    SMI_HIP(hipSetDevice(c->p.device));
+    for (int it = 0; it < nsteps; ++it) {
+        if (c->p.use_graph) {
+            // graph[k]: one step reading particle buffer k (captured once)
+            const int k = c->cur;
+            if (!c->graph[k]) {
+                hipGraph_t gr;
+                SMI_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                enqueue_step(c, false);
+                SMI_HIP(hipStreamEndCapture(c->st, &gr));
+                SMI_HIP(hipGraphInstantiate(&c->graph[k], gr, nullptr, nullptr, 0));
+                SMI_HIP(hipGraphDestroy(gr));
+            }
+            SMI_HIP(hipGraphLaunch(c->graph[k], c->st));
+        } else {
+            enqueue_step(c, false);
+        }
+        flip(c);
+    }
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_synchronize(smi_context *c)
+{
+    SMI_TRY
+    check_ctx(c);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    int e = 0;
+    SMI_HIP(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e)
+        throw Error(SMI_ERR_STATE, "a particle left its cluster stencil (moved more than one cell per step, "
+                                   "non-finite position or corrupt input); the step skipped it");
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_profile_step(smi_context *c, float *ms)
+{
+    SMI_TRY
+    check_ctx(c);
+    SMI_HIP(hipSetDevice(c->p.device));
+    enqueue_step(c, true);
+    flip(c);
+    SMI_HIP(hipEventSynchronize(c->ev[4]));
+    for (int k = 0; k < 4; ++k) SMI_HIP(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+    // ms[0] particles, ms[1] deterministic J reduction, ms[2] Maxwell, ms[3] sort
+    const float part = ms[0], jred = ms[1], maxw = ms[2], sort = ms[3];
+    ms[0] = part;
+    ms[1] = sort;
+    ms[2] = jred;
+    ms[3] = maxw;
+    return SMI_OK;
+    SMI_CATCH
+}
+
+void *smi_stream(smi_context *c) { return c ? (void *)c->st : nullptr; }
+
+int smi_energies(smi_context *c, double *out)
+{
+    SMI_TRY
+    check_ctx(c);
+    SMI_HIP(hipSetDevice(c->p.device));
+    std::vector<double> h(kReduceBlocks);
+    const double *fl[6] = {c->F[SMI_EX], c->F[SMI_EY], c->F[SMI_EZ], c->F[SMI_BXM], c->F[SMI_BYM], c->F[SMI_BZM]};
+    launch_sumsq(c->g, fl, 6, c->partials, kReduceBlocks, c->st);
+    SMI_HIP(hipMemcpyAsync(h.data(), c->partials, kReduceBlocks * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    double s = 0;
+    for (double v : h) s += v;
+    double cell_volume = 1.0;
+    for (int d = 0; d < c->g.nd; ++d) cell_volume *= c->g.dx[d];
+    out[0] = 0.5 * cell_volume * s;
+    for (size_t k = 0; k < c->sp.size(); ++k) {
+        SpeciesState &S = c->sp[k];
+        launch_ukin(S.view(c->cur), S.sb.perm, S.n, c->partials, kReduceBlocks, c->st);
+        SMI_HIP(hipMemcpyAsync(h.data(), c->partials, kReduceBlocks * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        SMI_HIP(hipStreamSynchronize(c->st));
+        double e = 0;
+        for (double v : h) e += v;
+        out[1 + k] = e * S.mass;
+    }
+    return SMI_OK;
+    SMI_CATCH
+}
+
+static void compute_rho(smi_context *c)
+{
+    launch_fill(c->rho, c->g.ncells, 0.0, c->st);
+    for (auto &S : c->sp) launch_rho(c->g, S.k, S.view(c->cur), S.sb.perm, S.n, c->rho, c->st);
+}
+
+int smi_rho(smi_context *c, double *gl)
+{
+    SMI_TRY
+    check_ctx(c);
+    SMI_HIP(hipSetDevice(c->p.device));
+    compute_rho(c);
+    SMI_HIP(hipMemcpyAsync(gl, c->rho, c->g.ncells * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_gauss(smi_context *c, double *gl)
+{
+    SMI_TRY
+    check_ctx(c);
+    SMI_HIP(hipSetDevice(c->p.device));
+    compute_rho(c);
+    launch_gauss(c->g, c->fields(), c->rho, c->st);
+    SMI_HIP(hipMemcpyAsync(gl, c->rho, c->g.ncells * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    return SMI_OK;
+    SMI_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// kernels.h -- host-side launchers of the HIP kernels (internal C++ API).
+#pragma once
+
+#include "common.h"
+
+namespace smi {
+
+// Records the text returned by smi_last_error() for the calling thread.
+void set_error(const std::string &msg);
+
+struct PartPtrs {
+    double *x, *y, *z, *px, *py, *pz, *w;
+};
+
+// Fields read by the gather (E and the centred B_m).
+struct GatherFields {
+    const double *Ex, *Ey, *Ez, *Bx, *By, *Bz;
+};
+
+struct JPtrs {
+    double *Jx, *Jy, *Jz;
+};
+
+// ---- particle step (particles.hip) ------------------------------------
+// One block per 4-cell cluster of every patch.  Reads the sorted particle
+// list through `perm` from A, writes the advanced particles to B in sorted
+// slot order, their packed destination keys (dir<<27 | global cell) and,
+// when `deposit`, the Esirkepov currents into J (LDS tile + FP64 atomics).
+// When !deposit, `scratch` (7*cap doubles) receives the old and the new
+// (unwrapped) positions and 1/gamma for the deterministic deposit.  A
+// particle found outside its cluster's stencil tile (corrupt input) is left
+// unchanged and raises *err.
+void launch_particles(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
+                      const PartPtrs &B, uint32_t *keys, double *scratch, int64_t cap,
+                      const int32_t *cell_off, const GatherFields &F, const JPtrs &J, bool deposit,
+                      int *err, hipStream_t st);
+
+// Deterministic deposit into per-patch ghosted J arrays (reference layout,
+// particle order of the patch).  `pj` holds npatches * (|Jx|+|Jy|+|Jz|).
+struct DetSpecies {
+    SpeciesK sp;
+    const int32_t *cell_off;
+    const double *scratch;  // old/new positions written by launch_particles
+    const double *pz, *w, *invgf;
+    int64_t cap;
+};
+void launch_det_deposit(const Geom &g, const DetSpecies *species, int nspecies, double *pj, int *err,
+                        hipStream_t st);
+// J ghost sum along each dimension (SyncVectorPatch::sumAllComponents) and
+// copy of the patch interiors into the global arrays.
+void launch_ghost_sum_and_fold(const Geom &g, double *pj, const JPtrs &J, hipStream_t st);
+int64_t patch_j_size(const Geom &g);  // doubles per patch (Jx+Jy+Jz)
+
+// ---- sort (sort.hip) --------------------------------------------------
+struct SortBufs {
+    int32_t *counts;   // ncells, kept at zero between calls
+    int32_t *cell_off; // ncells + 1
+    int32_t *perm;     // cap
+    int32_t *bsum;     // scan scratch
+    int64_t nbsum;
+};
+int64_t sort_bsum_size(int64_t ncells);
+// Counting sort of particles by destination cell, ties by (direction,
+// slot): fills cell_off and perm from keys[0..n).
+void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st);
+// Keys of freshly loaded particles (positions in A, slot order = input).
+void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
+
+// ---- initialisation (init.hip) ---------------------------------------
+// Particle i lives in cell cells[i/ppc] (or cell i/ppc when cells == null).
+void launch_init(const Geom &g, const PartPtrs &A, int64_t n, int ppc, const int32_t *cells, double weight,
+                 double sigma, double drift, uint64_t seed, hipStream_t st);
+
+// ---- fields (fields.hip) ----------------------------------------------
+struct FieldSet {
+    double *E[3], *B[3], *Bm[3], *J[3];
+};
+// saveMagneticFields + MA + MF + centerMagneticFields on the periodic grid.
+void launch_maxwell(const Geom &g, const FieldSet &f, hipStream_t st);
+void launch_fill(double *p, int64_t n, double v, hipStream_t st);
+void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+// Diagnostics.
+void launch_sumsq(const Geom &g, const double *const *fields, int nf, double *partials, int nblocks,
+                  hipStream_t st);
+void launch_ukin(const PartPtrs &A, const int32_t *perm, int64_t n, double *partials, int nblocks,
+                 hipStream_t st);
+void launch_rho(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm, int64_t n,
+                double *rho, hipStream_t st);
+void launch_gauss(const Geom &g, const FieldSet &f, double *rho_to_residual, hipStream_t st);
+
+}  // namespace smi

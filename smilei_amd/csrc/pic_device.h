@@ -1,0 +1,293 @@
+// pic_device.h -- device arithmetic shared by the batched kernels and the
+// per-operator entry points: Boris push, B-spline gathers and the Esirkepov
+// current contributions, each in the association order of the reference
+// operator it restates (the library is built with -ffp-contract=off).
+#pragma once
+
+#include "common.h"
+
+namespace smi {
+namespace dev {
+
+constexpr double kThird = 1. / 3.;
+
+// Tile geometry.  A cluster spans cells [c0, c0+4) along each axis; the
+// nearest primal node of a particle is in [c0, c0+4], the dual one in
+// [c0+1, c0+5].  Gather stencil +-H, deposit stencil +-(H+1).
+template <int ORDER> struct Tile {
+    static constexpr int H = ORDER / 2;
+    static constexpr int GO = H;             // gather tile starts at c0 - GO
+    static constexpr int GT = 4 + 2 * H + 2; // covers [c0-H, c0+5+H]
+    static constexpr int JO = H + 1;         // J tile starts at c0 - JO
+    static constexpr int JT = 4 + 2 * H + 3; // covers [c0-H-1, c0+4+H+1]
+    static constexpr int S = 2 * H + 3;      // Esirkepov stencil length
+};
+
+struct Cluster {
+    int p;          // patch id
+    int pc[3];      // patch coordinates
+    int G0[3];      // first global cell of the cluster
+    int64_t gbase;  // global cell key of the cluster's first cell
+};
+
+__device__ inline Cluster decode_cluster(const Geom &g, int b)
+{
+    Cluster c;
+    c.p = b / g.clpp;
+    const int cl = b % g.clpp;
+    int cc[3];
+    cc[0] = cl / (g.ncl[1] * g.ncl[2]);
+    cc[1] = (cl / g.ncl[2]) % g.ncl[1];
+    cc[2] = cl % g.ncl[2];
+    c.pc[0] = c.p / (g.npatch[1] * g.npatch[2]);
+    c.pc[1] = (c.p / g.npatch[2]) % g.npatch[1];
+    c.pc[2] = c.p % g.npatch[2];
+    for (int d = 0; d < 3; ++d) c.G0[d] = c.pc[d] * g.n[d] + 4 * cc[d];
+    c.gbase = (int64_t)c.p * g.cpp + (int64_t)cl * g.cells_per_cl;
+    return c;
+}
+
+// In-patch cell key (Cluster3D::CellKey, nvidiaParticles.cu:639-660) of a
+// position inside patch `pc`; clamped against rounding at the upper face.
+template <int ND>
+__device__ inline int32_t cell_key(const Geom &g, const int *pc, const double *x)
+{
+    int ic[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        int c = (int)floor(x[d] * (1.0 / g.dx[d])) - pc[d] * g.n[d];
+        c = c < 0 ? 0 : (c > g.n[d] - 1 ? g.n[d] - 1 : c);
+        ic[d] = c;
+    }
+    if constexpr (ND == 3) {
+        const int cl = ((ic[0] >> 2) * g.ncl[1] + (ic[1] >> 2)) * g.ncl[2] + (ic[2] >> 2);
+        return cl * 64 + 16 * (ic[0] & 3) + 4 * (ic[1] & 3) + (ic[2] & 3);
+    } else {
+        const int cl = (ic[0] >> 2) * g.ncl[1] + (ic[1] >> 2);
+        return cl * 16 + 4 * (ic[0] & 3) + (ic[1] & 3);
+    }
+}
+
+// Boris push (PusherBoris.cpp:80-127), FP64, reference association order.
+__device__ inline void boris(double qmh, double dt, const double *E, const double *B, double *u,
+                             double *dxp, double &invgf)
+{
+    double ax = qmh * E[0], ay = qmh * E[1], az = qmh * E[2];
+    const double ux = u[0] + ax, uy = u[1] + ay, uz = u[2] + az;
+    const double f = qmh / sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    const double tx = f * B[0], ty = f * B[1], tz = f * B[2];
+    const double inv_det = 1.0 / (1.0 + tx * tx + ty * ty + tz * tz);
+    ax += ((1.0 + tx * tx - ty * ty - tz * tz) * ux + 2.0 * (tx * ty + tz) * uy + 2.0 * (tz * tx - ty) * uz) *
+          inv_det;
+    ay += (2.0 * (tx * ty - tz) * ux + (1.0 - tx * tx + ty * ty - tz * tz) * uy + 2.0 * (ty * tz + tx) * uz) *
+          inv_det;
+    az += (2.0 * (tz * tx + ty) * ux + 2.0 * (ty * tz - tx) * uy + (1.0 - tx * tx - ty * ty + tz * tz) * uz) *
+          inv_det;
+    double gi = 1. / sqrt(1.0 + ax * ax + ay * ay + az * az);
+    invgf = gi;
+    u[0] = ax;
+    u[1] = ay;
+    u[2] = az;
+    gi *= dt;
+    dxp[0] = ax * gi;
+    dxp[1] = ay * gi;
+    dxp[2] = az * gi;
+}
+
+// Gather from an LDS tile of width T: sum_{a,b,c} ((cx*cy)*cz)*f, a outer.
+template <int H, int T>
+__device__ inline double gather3(const double *cx, const double *cy, const double *cz, const double *t,
+                                 int i, int j, int k)
+{
+    double r = 0.;
+#pragma unroll
+    for (int a = -H; a <= H; ++a)
+#pragma unroll
+        for (int b = -H; b <= H; ++b)
+#pragma unroll
+            for (int c = -H; c <= H; ++c)
+                r += cx[a + H] * cy[b + H] * cz[c + H] * t[((i + a) * T + (j + b)) * T + (k + c)];
+    return r;
+}
+
+template <int H, int T>
+__device__ inline double gather2(const double *cx, const double *cy, const double *t, int i, int j)
+{
+    double r = 0.;
+#pragma unroll
+    for (int a = -H; a <= H; ++a)
+#pragma unroll
+        for (int b = -H; b <= H; ++b) r += cx[a + H] * cy[b + H] * t[(i + a) * T + (j + b)];
+    return r;
+}
+
+// Same gathers on a strided array (the reference's per-patch Field3D/2D).
+template <int H>
+__device__ inline double gather3s(const double *cx, const double *cy, const double *cz, const double *f, int i,
+                                  int j, int k, int ny, int nz)
+{
+    double r = 0.;
+#pragma unroll
+    for (int a = -H; a <= H; ++a)
+#pragma unroll
+        for (int b = -H; b <= H; ++b)
+#pragma unroll
+            for (int c = -H; c <= H; ++c)
+                r += cx[a + H] * cy[b + H] * cz[c + H] * f[((int64_t)(i + a) * ny + (j + b)) * nz + (k + c)];
+    return r;
+}
+
+template <int H>
+__device__ inline double gather2s(const double *cx, const double *cy, const double *f, int i, int j, int ny)
+{
+    double r = 0.;
+#pragma unroll
+    for (int a = -H; a <= H; ++a)
+#pragma unroll
+        for (int b = -H; b <= H; ++b) r += cx[a + H] * cy[b + H] * f[(int64_t)(i + a) * ny + (j + b)];
+    return r;
+}
+
+// Esirkepov shapes.  S0 from the old offset, S1 from the new position,
+// shifted by ip - ipo; DS = S1 - S0 (Projector3D2Order.cpp:92-148).
+template <int ORDER>
+__device__ inline void esirkepov_shapes(double dold, double pn_new, int ipo, double *S0, double *S1,
+                                        double *DS)
+{
+    constexpr int S = Tile<ORDER>::S;
+    S0[0] = 0.;
+    spline<ORDER>(dold, &S0[1]);
+    S0[S - 1] = 0.;
+#pragma unroll
+    for (int i = 0; i < S; ++i) S1[i] = 0.;
+    const int ip = (int)round(pn_new);
+    const double d = pn_new - (double)ip;
+    double w[5];
+    spline<ORDER>(d, w);
+    const int sh = ip - ipo;  // -1, 0 or 1
+#pragma unroll
+    for (int i = 0; i < ORDER + 1; ++i) {
+        // S1[sh + 1 + i] = w[i] with compile-time indices only
+#pragma unroll
+        for (int q = -1; q <= 1; ++q)
+            if (q == sh) S1[q + 1 + i] = w[i];
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) DS[i] = S1[i] - S0[i];
+}
+
+// Transverse Esirkepov weight W(b,c) for the current along axis a, with
+// (b,c) the two other axes in cyclic order (Jx:(y,z), Jy:(z,x), Jz:(x,y)),
+// Projector3D2Order.cpp:176-330.  With S0 zero at both stencil ends this
+// single expression rounds exactly like the reference's four special-cased
+// blocks.
+__device__ inline double weight3(const double *S0b, const double *DSb, const double *S0c,
+                                 const double *DSc, int jb, int jc)
+{
+    return S0b[jb] * S0c[jc] + 0.5 * DSb[jb] * S0c[jc] + 0.5 * DSc[jc] * S0b[jb] +
+           kThird * DSb[jb] * DSc[jc];
+}
+
+// Walk the 3D Esirkepov contributions of one particle and call
+// add(component, i, j, k, value) with (i,j,k) offsets from (ipo, jpo, kpo)
+// minus (H+1).  Order 2: tmp = cr*W, acc -= DS*tmp; order 4:
+// acc -= (cr*DS)*W (Projector3D4Order.cpp:190-232).
+template <int ORDER, class Add>
+__device__ inline void esirkepov3d(const double (&S0)[3][7], const double (&DS)[3][7], const double *cr,
+                                   Add add)
+{
+    constexpr int S = Tile<ORDER>::S;
+    // Jx: prefix along x over rows i = 1..S-1, transverse (y=j, z=k)
+    for (int j = 0; j < S; ++j)
+        for (int k = 0; k < S; ++k) {
+            const double W = weight3(S0[1], DS[1], S0[2], DS[2], j, k);
+            double acc = 0.;
+            if constexpr (ORDER == 2) {
+                const double t = cr[0] * W;
+                for (int i = 1; i < S; ++i) {
+                    acc -= DS[0][i - 1] * t;
+                    add(0, i, j, k, acc);
+                }
+            } else {
+                for (int i = 1; i < S; ++i) {
+                    acc -= cr[0] * DS[0][i - 1] * W;
+                    add(0, i, j, k, acc);
+                }
+            }
+        }
+    // Jy: prefix along y, transverse (z=k, x=i)
+    for (int i = 0; i < S; ++i)
+        for (int k = 0; k < S; ++k) {
+            const double W = weight3(S0[2], DS[2], S0[0], DS[0], k, i);
+            double acc = 0.;
+            if constexpr (ORDER == 2) {
+                const double t = cr[1] * W;
+                for (int j = 1; j < S; ++j) {
+                    acc -= DS[1][j - 1] * t;
+                    add(1, i, j, k, acc);
+                }
+            } else {
+                for (int j = 1; j < S; ++j) {
+                    acc -= cr[1] * DS[1][j - 1] * W;
+                    add(1, i, j, k, acc);
+                }
+            }
+        }
+    // Jz: prefix along z, transverse (x=i, y=j)
+    for (int i = 0; i < S; ++i)
+        for (int j = 0; j < S; ++j) {
+            const double W = weight3(S0[0], DS[0], S0[1], DS[1], i, j);
+            double acc = 0.;
+            if constexpr (ORDER == 2) {
+                const double t = cr[2] * W;
+                for (int k = 1; k < S; ++k) {
+                    acc -= DS[2][k - 1] * t;
+                    add(2, i, j, k, acc);
+                }
+            } else {
+                for (int k = 1; k < S; ++k) {
+                    acc -= cr[2] * DS[2][k - 1] * W;
+                    add(2, i, j, k, acc);
+                }
+            }
+        }
+}
+
+// 2D3V Esirkepov of Projector2D2Order::currents (Projector2D2Order.cpp:
+// 131-188): Jx prefix along x per column j, Jy prefix along y per row i,
+// Jz projected with the time-averaged shape.  add(c, i, j, v).
+template <class Add>
+__device__ inline void esirkepov2d(const double (&S0)[3][7], const double (&S1)[3][7],
+                                   const double (&DS)[3][7], double crx, double cry, double crz, Add add)
+{
+    double tJx[5] = {0., 0., 0., 0., 0.};
+    {
+        const double h2 = 0.5 * S1[0][0], h3 = S1[0][0];
+        add(2, 0, 0, crz * (S1[1][0] * h3));
+        double acc = 0.;
+        const double ty = S0[0][0] + 0.5 * DS[0][0];
+        for (int j = 1; j < 5; ++j) {
+            acc -= cry * DS[1][j - 1] * ty;
+            add(1, 0, j, acc);
+            add(2, 0, j, crz * (S0[1][j] * h2 + S1[1][j] * h3));
+        }
+    }
+    for (int i = 1; i < 5; ++i) {
+        tJx[0] -= crx * DS[0][i - 1] * (0.5 * DS[1][0]);
+        add(0, i, 0, tJx[0]);
+        const double h2 = 0.5 * S1[0][i] + S0[0][i], h3 = 0.5 * S0[0][i] + S1[0][i];
+        add(2, i, 0, crz * (S1[1][0] * h3));
+        double acc = 0.;
+        const double ty = S0[0][i] + 0.5 * DS[0][i];
+        for (int j = 1; j < 5; ++j) {
+            tJx[j] -= crx * DS[0][i - 1] * (S0[1][j] + 0.5 * DS[1][j]);
+            add(0, i, j, tJx[j]);
+            acc -= cry * DS[1][j - 1] * ty;
+            add(1, i, j, acc);
+            add(2, i, j, crz * (S0[1][j] * h2 + S1[1][j] * h3));
+        }
+    }
+}
+
+}  // namespace dev
+}  // namespace smi

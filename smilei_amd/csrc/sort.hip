@@ -1,0 +1,228 @@
+// sort.hip -- per-step cell sort and inter-patch exchange of particles.
+//
+// Replaces the reference's leaving-particle extraction, neighbour exchange
+// and per-patch sort (Particles::copyLeavingParticlesToBuffers
+// Particles.cpp:1308, Patch::exchParticles/cornersParticles Patch.cpp:
+// 670-801, nvidiaParticles::sortParticleByKey nvidiaParticles.cu:1217-1247)
+// with one device-wide counting sort by destination cell, since the push
+// kernel already computed every particle's destination patch and cell:
+//   1. histogram of destination cells          (int32 atomics)
+//   2. exclusive scan -> cell_off               (3-phase block scan)
+//   3. scatter of slot indices into buckets     (atomic count-down, which
+//      also returns the histogram to zero for the next step)
+//   4. per-bucket insertion sort by (direction of motion, source slot), the
+//      canonical order of particles inside a cell.
+// The particle data itself is not moved here: the next step's particle
+// kernel reads its input through `perm`.
+#include "kernels.h"
+
+namespace smi {
+
+namespace {
+
+constexpr uint32_t kKeyMask = (1u << 27) - 1;
+
+__global__ void k_hist(const uint32_t *__restrict__ keys, int64_t n, int32_t *__restrict__ counts)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    atomicAdd(&counts[keys[i] & kKeyMask], 1);
+}
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Block-wide exclusive scan of one int per thread; returns the block total.
+__device__ inline int block_exclusive_scan(int v, int &total)
+{
+    __shared__ int warp_tot[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) warp_tot[wid] = incl;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wid) base += warp_tot[w];
+        tot += warp_tot[w];
+    }
+    __syncthreads();
+    total = tot;
+    return base + incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_reduce(const int32_t *__restrict__ in, int64_t n, int32_t *__restrict__ bsum)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + (int64_t)k * kScanThreads + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    block_exclusive_scan(s, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Single-block exclusive scan of the block sums (in place).
+__global__ void __launch_bounds__(kScanThreads) k_scan_bsum(int32_t *__restrict__ bsum, int64_t nb)
+{
+    int carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kScanTile) {
+        int v[kScanItems];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t i = b0 + (int64_t)threadIdx.x * kScanItems + k;
+            v[k] = i < nb ? bsum[i] : 0;
+            s += v[k];
+        }
+        int tot;
+        int ex = block_exclusive_scan(s, tot) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t i = b0 + (int64_t)threadIdx.x * kScanItems + k;
+            if (i < nb) bsum[i] = ex;
+            ex += v[k];
+        }
+        carry += tot;
+    }
+}
+
+// out[i] = bsum[block] + exclusive prefix inside the block; out[n] = total.
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_apply(const int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bsum,
+             int32_t *__restrict__ out)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_exclusive_scan(s, tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = ex;
+        if (i == n - 1) out[n] = ex + v[k];
+        ex += v[k];
+    }
+}
+
+__global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
+                          int32_t *__restrict__ counts, int32_t *__restrict__ perm)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = keys[i] & kKeyMask;
+    const int c = atomicSub(&counts[g], 1);
+    perm[cell_off[g] + c - 1] = (int32_t)i;
+}
+
+// Canonical order inside a cell: by direction of motion, then source slot.
+__global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
+                             int32_t *__restrict__ perm)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ncells) return;
+    const int a = cell_off[g], b = cell_off[g + 1];
+    if (b - a < 2) return;
+    for (int i = a + 1; i < b; ++i) {
+        const int32_t v = perm[i];
+        const uint64_t kv = ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)v;
+        int j = i - 1;
+        while (j >= a) {
+            const int32_t u = perm[j];
+            const uint64_t ku = ((uint64_t)(keys[u] >> 27) << 32) | (uint32_t)u;
+            if (ku <= kv) break;
+            perm[j + 1] = u;
+            --j;
+        }
+        perm[j + 1] = v;
+    }
+}
+
+// Patch and cell key of loaded particles; all get direction 13 ("stay") so
+// that the canonical order of a cell is the input order.
+template <int ND>
+__global__ void k_bin_initial(Geom g, PartPtrs A, int64_t n, uint32_t *__restrict__ keys)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x[3] = {A.x[i], A.y[i], ND == 3 ? A.z[i] : 0.};
+    int pc[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
+    for (int d = 0; d < ND; ++d) {
+        int c = (int)floor(x[d] / g.plen[d]);
+        c = c < 0 ? 0 : (c > g.npatch[d] - 1 ? g.npatch[d] - 1 : c);
+        while (c > 0 && x[d] < (double)c * g.plen[d]) --c;
+        while (c < g.npatch[d] - 1 && x[d] >= (double)(c + 1) * g.plen[d]) ++c;
+        pc[d] = c;
+        int k = (int)floor(x[d] * (1.0 / g.dx[d])) - c * g.n[d];
+        ic[d] = k < 0 ? 0 : (k > g.n[d] - 1 ? g.n[d] - 1 : k);
+    }
+    int key;
+    if constexpr (ND == 3) {
+        const int cl = ((ic[0] >> 2) * g.ncl[1] + (ic[1] >> 2)) * g.ncl[2] + (ic[2] >> 2);
+        key = cl * 64 + 16 * (ic[0] & 3) + 4 * (ic[1] & 3) + (ic[2] & 3);
+    } else {
+        const int cl = (ic[0] >> 2) * g.ncl[1] + (ic[1] >> 2);
+        key = cl * 16 + 4 * (ic[0] & 3) + (ic[1] & 3);
+    }
+    const int p = (pc[0] * g.npatch[1] + pc[1]) * g.npatch[2] + pc[2];
+    keys[i] = (13u << 27) | (uint32_t)((int64_t)p * g.cpp + key);
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+int64_t sort_bsum_size(int64_t ncells) { return (ncells + kScanTile - 1) / kScanTile + 1; }
+
+void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st)
+{
+    const int64_t nc = g.ncells;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_hist, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, sb.counts);
+        SMI_LAUNCH_CHECK();
+    }
+    const int64_t nb = (nc + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, st, (const int32_t *)sb.counts, nc,
+                       sb.bsum);
+    SMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kScanThreads), 0, st, sb.bsum, nb);
+    SMI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, st, (const int32_t *)sb.counts, nc,
+                       (const int32_t *)sb.bsum, sb.cell_off);
+    SMI_LAUNCH_CHECK();
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
+                           sb.counts, sb.perm);
+        SMI_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
+                           (const int32_t *)sb.cell_off, sb.perm);
+        SMI_LAUNCH_CHECK();
+    }
+}
+
+void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st)
+{
+    if (n == 0) return;
+    if (g.nd == 3) hipLaunchKernelGGL(k_bin_initial<3>, dim3(nblk(n, 256)), dim3(256), 0, st, g, A, n, keys);
+    else hipLaunchKernelGGL(k_bin_initial<2>, dim3(nblk(n, 256)), dim3(256), 0, st, g, A, n, keys);
+    SMI_LAUNCH_CHECK();
+}
+
+}  // namespace smi
