@@ -113,9 +113,9 @@ def test_push_boris_bit_exact(ndim):
     ref = O.push_boris(ndim, q, m, dt, *x, *u, Ep, Bp)
     dx_ = [dev(a) for a in x]
     du = [dev(a) for a in u]
+    gE, gB = dev(Ep), dev(Bp)  # kept alive until the kernel has run
     inv = torch.empty(n, dtype=torch.float64, device="cuda")
-    _lib.check(L.smi_push_boris(ndim, q, m, dt, n, *[ptr(t) for t in dx_ + du], ptr(dev(Ep)), ptr(dev(Bp)),
-                                ptr(inv), None))
+    _lib.check(L.smi_push_boris(ndim, q, m, dt, n, *[ptr(t) for t in dx_ + du], ptr(gE), ptr(gB), ptr(inv), None))
     for a, b in zip([host(t) for t in dx_ + du] + [host(inv)], ref):
         np.testing.assert_array_equal(a, b)
 
@@ -151,9 +151,9 @@ def test_currents_3d(order, deterministic):
     q = -1.0
     O.proj3d(order, J, np3, dom, dx, dt, 1 / (dx[0] * dx[1] * dx[2]), q, *new, w, iold, dold)
     gJ = [torch.zeros(s, dtype=torch.float64, device="cuda") for s in shapes]
+    ins = [dev(a) for a in new + [w, iold, dold]]  # kept alive until the kernel has run
     _lib.check(L.smi_currents_3d(order, *[ptr(t) for t in gJ], I3(np3), I3(dom), D3(dx), dt, q, n,
-                                 *[ptr(dev(a)) for a in new], ptr(dev(w)), ptr(dev(iold)), ptr(dev(dold)),
-                                 deterministic, None))
+                                 *[ptr(t) for t in ins], deterministic, None))
     for a, b in zip(gJ, J):
         if deterministic:
             np.testing.assert_array_equal(host(a), b)
@@ -186,9 +186,9 @@ def test_currents_2d(deterministic):
     J = [np.zeros(s) for s in shapes]
     O.proj2d(2, J, np2, dom, dx, dt, 1 / (dx[0] * dx[1]), -1.0, *new, pz, w, invgf, iold, dold)
     gJ = [torch.zeros(s, dtype=torch.float64, device="cuda") for s in shapes]
+    ins = [dev(a) for a in new + [pz, w, invgf, iold, dold]]  # kept alive until the kernel has run
     _lib.check(L.smi_currents_2d(2, *[ptr(t) for t in gJ], (ctypes.c_int * 2)(*np2), (ctypes.c_int * 2)(*dom),
-                                 (ctypes.c_double * 2)(*dx), dt, -1.0, n, *[ptr(dev(a)) for a in new],
-                                 ptr(dev(pz)), ptr(dev(w)), ptr(dev(invgf)), ptr(dev(iold)), ptr(dev(dold)),
+                                 (ctypes.c_double * 2)(*dx), dt, -1.0, n, *[ptr(t) for t in ins],
                                  deterministic, None))
     for a, b in zip(gJ, J):
         if deterministic:
@@ -211,6 +211,7 @@ def test_maxwell_3d_patch_bit_exact():
     gE = [dev(a) for a in E]
     gB = [dev(a) for a in B]
     gJ = [dev(a) for a in Jc]
+    torch.cuda.synchronize()
     O.yee3d_ma(E, B, Jc, np3, dt, dx)
     O.yee3d_mf(B, E, np3, dt, dx)
     _lib.check(L.smi_maxwell_ampere_3d(*[ptr(t) for t in gE + gB + gJ], I3(np3), dt, D3(dx), None))
