@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Particle-pushes/s of the PIC time step on one MI355X.
+
+Workload (the plain-FP64 3D thermal plasma): 256x128x128 cells, dx = 0.2,
+dt = 0.95*CFL, 16^3-cell patches (1024 patches), electrons and protons
+(m = 1836) at 8 particles/cell each = 67,108,864 particles, uniform random
+positions, Gaussian momenta sigma = 0.05 m_e c (protons: same temperature,
+sigma = 0.05/sqrt(1836) in m_p c), 2nd-order shapes, Yee, periodic.  The
+particles are initialised in HBM; one "step" is the full PIC step (gather,
+Boris push, Esirkepov deposit, cell sort + patch exchange, Yee update).
+
+Prints one JSON line.  `value` = particles * steps / wall time of the timed
+steps.  `roofline` rates the dominant kernel (the fused gather-push-deposit
+kernel) against HBM: algorithmic bytes = 108 B per particle push (read
+x,y,z,ux,uy,uz,w = 56 B; write x,y,z,ux,uy,uz = 48 B; write the cell key
+4 B) divided by that kernel's device time, measured with HIP events on the
+library's stream over the timed steps.  `cpu_baseline` times the CPU
+oracle (OpenMP over patches, the reference's CPU structure) on a bounded
+sub-box of the same plasma.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PUSH = {"f64": 108.0}
+
+
+def cfl_dt(dx, factor=0.95):
+    return factor / np.sqrt(sum(1.0 / d ** 2 for d in dx))
+
+
+def workload():
+    shape = (256, 128, 128)
+    dx = (0.2, 0.2, 0.2)
+    return dict(shape=shape, patch=(16, 16, 16), dx=dx, dt=cfl_dt(dx), ppc=8,
+                species=[("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))])
+
+
+def run_gpu(args, W):
+    import smilei_amd as sa
+    sim = sa.Simulation("3Dcartesian", 2, W["shape"], W["patch"], W["dx"], W["dt"],
+                        precision=sa.PLAIN_FP64, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
+    ncell = int(np.prod(W["shape"]))
+    for k, (name, m, q, sig) in enumerate(W["species"]):
+        sid = sim.add_species(name, m, q, ncell * W["ppc"])
+        sim.init_species(sid, W["ppc"], 1.0, sig, seed=1 + k)
+    npart = sum(sim.num_particles(s) for s in range(len(W["species"])))
+    sim.step(args.warmup)
+    sim.synchronize()
+    # timed region: eager steps with HIP events around each kernel class
+    acc = {"particles": 0.0, "sort": 0.0, "j_reduce": 0.0, "maxwell": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ms = sim.profile_step()
+        for k in acc:
+            acc[k] += ms[k]
+    sim.synchronize()
+    t1 = time.perf_counter()
+    # graph-replayed steps (the production path) for comparison
+    tg0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.synchronize()
+    tg1 = time.perf_counter()
+    e = sim.energies()
+    sim.close()
+    return dict(npart=npart, wall=t1 - t0, wall_graph=tg1 - tg0, kernel_ms=acc, energies=e.tolist())
+
+
+def run_cpu_baseline(args, W):
+    """The oracle on a 64^3-cell sub-box of the same plasma (same dx, dt,
+    patch size and ppc): 4.19M particles, a few steps, all host cores."""
+    from oracle import pyoracle as O
+    from tests.plasma import thermal
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    shape = (64, 64, 64)
+    S = O.OracleSim(3, 2, shape, W["patch"], W["dx"], W["dt"])
+    n = 0
+    for k, (name, m, q, sig) in enumerate(W["species"]):
+        p = thermal(shape, W["dx"], W["ppc"], sig, 100 + k)
+        sid = S.add_species(m, q)
+        S.set_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+        n += len(p["x"])
+    S.step(1)  # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        S.step(1)
+        steps += 1
+        if time.perf_counter() - t0 > args.cpu_seconds or steps >= 50:
+            break
+    dt = time.perf_counter() - t0
+    rate = n * steps / dt
+    return dict(value=rate, unit="particle-pushes/s", cores=threads, kind="port",
+                sample="CPU oracle, %dx%dx%d-cell sub-box of the same plasma (16^3 patches, 8+8 ppc, %d particles), "
+                       "%d steps, OpenMP over patches; %.1f ns*core/push" % (shape + (n, steps, 1e9 * dt * threads /
+                                                                                  (n * steps))))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the PIC step is not sharded across GPUs here); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    W = workload()
+    g = run_gpu(args, W)
+    npart, K = g["npart"], args.steps
+    value = npart * K / g["wall"]
+    t_part = g["kernel_ms"]["particles"] / 1e3 / K  # s per step, both species
+    achieved = BYTES_PER_PUSH["f64"] * npart / t_part / 1e9
+    out = {
+        "metric": "particle-pushes/s (gather+Boris+Esirkepov) 1xMI355X, mixed vs FP64; %HBM peak",
+        "value": value,
+        "unit": "particle-pushes/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * g["wall"] / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded thermal plasma initialised in HBM)",
+        "config": {
+            "workload": "3D thermal plasma 256x128x128 cells, dx=0.2, 16^3-cell patches, e-/p+ 8 ppc each "
+                        "(%d particles), order 2, Yee, periodic, plain FP64 form, atomic deposit" % npart,
+            "particles": npart,
+            "ms_per_step_graph": 1e3 * g["wall_graph"] / K,
+            "kernel_ms_per_step": {k: v / K for k, v in g["kernel_ms"].items()},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_particles<3,2,deposit> (fused gather+Boris+key+Esirkepov), 108 B/push algorithmic",
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = run_cpu_baseline(args, W)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
