@@ -139,7 +139,7 @@ void enqueue_step(smi_context *c, bool timed)
         for (int q = 0; q < 3; ++q) launch_fill(f.J[q], g.ncells, 0.0, c->st);  // restartRhoJ
     for (auto &s : c->sp) {
         launch_particles(g, s.k, s.view(a), s.sb.perm, s.view(1 - a), s.keys, s.scratch, s.cap, s.sb.cell_off, GF, J,
-                         !det, c->err, c->st);
+                         !det, c->err, s.sb.counts, c->st);
     }
     if (timed) SMI_HIP(hipEventRecord(c->ev[1], c->st));
     if (det && !c->sp.empty()) {
@@ -149,7 +149,7 @@ void enqueue_step(smi_context *c, bool timed)
     if (timed) SMI_HIP(hipEventRecord(c->ev[2], c->st));
     launch_maxwell(g, f, c->st);
     if (timed) SMI_HIP(hipEventRecord(c->ev[3], c->st));
-    for (auto &s : c->sp) launch_sort(g, s.keys, s.n, s.sb, c->st);
+    for (auto &s : c->sp) launch_sort(g, s.keys, s.n, s.sb, c->st, /*counted=*/true);
     if (timed) SMI_HIP(hipEventRecord(c->ev[4], c->st));
 }
 

@@ -29,11 +29,12 @@ struct JPtrs {
 // When !deposit, `scratch` (7*cap doubles) receives the old and the new
 // (unwrapped) positions and 1/gamma for the deterministic deposit.  A
 // particle found outside its cluster's stencil tile (corrupt input) is left
-// unchanged and raises *err.
+// unchanged and raises *err.  The kernel also accumulates the histogram of
+// destination cells into `counts` (zero on entry) for launch_sort.
 void launch_particles(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
                       const PartPtrs &B, uint32_t *keys, double *scratch, int64_t cap,
                       const int32_t *cell_off, const GatherFields &F, const JPtrs &J, bool deposit,
-                      int *err, hipStream_t st);
+                      int *err, int32_t *counts, hipStream_t st);
 
 // Deterministic deposit into per-patch ghosted J arrays (reference layout,
 // particle order of the patch).  `pj` holds npatches * (|Jx|+|Jy|+|Jz|).
@@ -61,8 +62,11 @@ struct SortBufs {
 };
 int64_t sort_bsum_size(int64_t ncells);
 // Counting sort of particles by destination cell, ties by (direction,
-// slot): fills cell_off and perm from keys[0..n).
-void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st);
+// slot): fills cell_off and perm from keys[0..n).  With counted = true the
+// histogram of destination cells is already in sb.counts (the particle
+// kernel builds it); otherwise it is computed from the keys first.
+void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st,
+                 bool counted = false);
 // Keys of freshly loaded particles (positions in A, slot order = input).
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
 

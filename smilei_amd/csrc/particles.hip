@@ -34,10 +34,11 @@ using namespace dev;
 // Fused particle kernel.
 // ---------------------------------------------------------------------------
 template <int ND, int ORDER, bool DEPOSIT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 3)
 k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, PartPtrs B,
             uint32_t *__restrict__ keys, double *__restrict__ scratch, int64_t cap,
-            const int32_t *__restrict__ cell_off, GatherFields F, JPtrs J, int *__restrict__ err)
+            const int32_t *__restrict__ cell_off, GatherFields F, JPtrs J, int *__restrict__ err,
+            int32_t *__restrict__ counts)
 {
     using TL = Tile<ORDER>;
     constexpr int H = TL::H, GT = TL::GT, JT = TL::JT, S = TL::S;
@@ -45,6 +46,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     constexpr int JT3 = ND == 3 ? JT * JT * JT : JT * JT;
     __shared__ double sF[6 * GT3];
     __shared__ double sJ[DEPOSIT ? 3 * JT3 : 1];
+    __shared__ int sHist[64];  // destination-cell counts of this cluster's own cells
 
     const Cluster cl = decode_cluster(g, blockIdx.x);
     const int s = cell_off[cl.gbase], e = cell_off[cl.gbase + g.cells_per_cl];
@@ -72,6 +74,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     }
     if constexpr (DEPOSIT)
         for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.;
+    if (threadIdx.x < 64) sHist[threadIdx.x] = 0;
     __syncthreads();
 
     double pmin[3], pmax[3];
@@ -81,7 +84,16 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
         pmax[d] = (double)(cl.pc[d] + 1) * g.plen[d];
     }
 
-    for (int jslot = s + threadIdx.x; jslot < e; jslot += blockDim.x) {
+    // Particles of the cluster are sorted by cell (about ppc per cell).  Lane l
+    // of every wavefront takes slot l*q + row, so the 64 lanes of a wave work
+    // in different cells and their LDS atomics rarely hit the same address.
+    const int N = e - s;
+    const int q = DEPOSIT ? (N + 63) >> 6 : 1;
+    const int span = DEPOSIT ? 64 * q : N;
+    for (int kk = threadIdx.x; kk < span; kk += blockDim.x) {
+        const int rel = DEPOSIT ? (kk & 63) * q + (kk >> 6) : kk;
+        if (rel >= N) continue;
+        const int jslot = s + rel;
         const int i = perm[jslot];
         double x[3] = {A.x[i], A.y[i], ND == 3 ? A.z[i] : 0.};
         double u[3] = {A.px[i], A.py[i], A.pz[i]};
@@ -111,6 +123,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
             // not in this cluster (corrupt state): leave it untouched, flag it
             atomicOr(err, 1);
             keys[jslot] = (13u << 27) | (uint32_t)(cl.gbase);
+            atomicAdd(&sHist[0], 1);
             B.x[jslot] = x[0];
             B.y[jslot] = x[1];
             if constexpr (ND == 3) B.z[jslot] = x[2];
@@ -166,6 +179,11 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
         const uint32_t gkey = (uint32_t)((int64_t)dp * g.cpp + cell_key<ND>(g, dpc, xw));
         const uint32_t dir = (uint32_t)((m[0] + 1) * 9 + (m[1] + 1) * 3 + (m[2] + 1));
         keys[jslot] = (dir << 27) | gkey;
+        // histogram of destination cells for the sort: own-cluster cells in
+        // LDS, particles leaving the cluster straight to the global counts
+        const int64_t rel_cell = (int64_t)gkey - cl.gbase;
+        if (rel_cell >= 0 && rel_cell < g.cells_per_cl) atomicAdd(&sHist[rel_cell], 1);
+        else atomicAdd(&counts[gkey], 1);
         B.x[jslot] = xw[0];
         B.y[jslot] = xw[1];
         if constexpr (ND == 3) B.z[jslot] = xw[2];
@@ -184,20 +202,26 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
         } else {
             // ---- Esirkepov currents into the LDS tile
             const double cw = sp.cw_scale * w;
-            double S0[3][7], S1[3][7], DS[3][7];
-#pragma unroll
-            for (int d = 0; d < ND; ++d)
-                esirkepov_shapes<ORDER>(dold[d], xn[d] * g.dinv[d], ip[d], S0[d], S1[d], DS[d]);
             if constexpr (ND == 3) {
+                double S0[3][ORDER + 2], DS[3][ORDER + 2];
+                int w0[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    esirkepov_window<ORDER>(dold[d], xn[d] * g.dinv[d], ip[d], S0[d], DS[d], w0[d]);
                 const double cr[3] = {cw * (g.dx[0] / g.dt), cw * (g.dx[1] / g.dt), cw * (g.dx[2] / g.dt)};
-                // tile coords of stencil offset 0 along each axis
+                // tile coords of stencil index 0 along each axis
                 const int o0 = ip[0] - (H + 1) - (cl.G0[0] - TL::JO);
                 const int o1 = ip[1] - (H + 1) - (cl.G0[1] - TL::JO);
                 const int o2 = ip[2] - (H + 1) - (cl.G0[2] - TL::JO);
-                esirkepov3d<ORDER>(S0, DS, cr, [&](int c, int a, int b, int k, double v) {
-                    if (v != 0.) atomicAdd(&sJ[c * JT3 + ((o0 + a) * JT + (o1 + b)) * JT + (o2 + k)], v);
+                double *sJb = sJ + (o0 * JT + o1) * JT + o2;
+                esirkepov3d_win<ORDER>(S0, DS, w0, cr, [&](int c, int a, int b, int k, double v) {
+                    atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
                 });
             } else {
+                double S0[3][7], S1[3][7], DS[3][7];
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    esirkepov_shapes<ORDER>(dold[d], xn[d] * g.dinv[d], ip[d], S0[d], S1[d], DS[d]);
                 const double crx = cw * (g.dx[0] / g.dt), cry = cw * (g.dx[1] / g.dt);
                 const double crz = cw * kThird * u[2] * invgf;
                 const int o0 = ip[0] - 2 - (cl.G0[0] - TL::JO);
@@ -209,8 +233,12 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
         }
     }
 
+    __syncthreads();
+    if (threadIdx.x < g.cells_per_cl) {
+        const int c = sHist[threadIdx.x];
+        if (c) atomicAdd(&counts[cl.gbase + threadIdx.x], c);
+    }
     if constexpr (DEPOSIT) {
-        __syncthreads();
         double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
         for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
             const double v = sJ[t];
@@ -452,12 +480,12 @@ int64_t patch_j_size(const Geom &g) { return pj_layout(g).per; }
 void launch_particles(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
                       const PartPtrs &B, uint32_t *keys, double *scratch, int64_t cap,
                       const int32_t *cell_off, const GatherFields &F, const JPtrs &J, bool deposit,
-                      int *err, hipStream_t st)
+                      int *err, int32_t *counts, hipStream_t st)
 {
     const int nb = g.npatches * g.clpp;
     const dim3 grid(nb), block(256);
 #define SMI_PK(ND, O, D) hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, sp, A, perm, B, keys, \
-                                            scratch, cap, cell_off, F, J, err)
+                                            scratch, cap, cell_off, F, J, err, counts)
     if (g.nd == 3 && g.order == 2) {
         if (deposit) SMI_PK(3, 2, true);
         else SMI_PK(3, 2, false);

@@ -253,6 +253,95 @@ __device__ inline void esirkepov3d(const double (&S0)[3][7], const double (&DS)[
         }
 }
 
+// The part of a particle's Esirkepov stencil that can be nonzero: along
+// each axis S0 (old position) and S1 (new position, shifted by the cell
+// change sh in {-1,0,1}) both lie inside a window of ORDER+2 consecutive
+// entries of the reference's ORDER+3-entry stencil, starting at w0 = 0 when
+// sh < 0 and at w0 = 1 otherwise.  Outside the window S0 = DS = 0 exactly,
+// so the skipped columns contribute exact zeros, and the prefix sums along
+// the current's axis start from an exact zero: the values added below are
+// bit-identical to the reference's.  The window arrays are filled with
+// compile-time indices only (selects, no dynamic register indexing).
+template <int ORDER>
+__device__ inline void esirkepov_window(double dold, double pn_new, int ipo, double *S0w, double *DSw, int &w0)
+{
+    constexpr int W = ORDER + 2;
+    double s[ORDER + 1], nw[ORDER + 1];
+    spline<ORDER>(dold, s);
+    const int ip = (int)round(pn_new);
+    spline<ORDER>(pn_new - (double)ip, nw);
+    const int sh = ip - ipo;
+    w0 = sh < 0 ? 0 : 1;
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        const double s0_w0 = m == 0 ? 0. : s[m - 1];     // window starting at stencil 0
+        const double s0_w1 = m <= ORDER ? s[m] : 0.;     // window starting at stencil 1
+        const double s1_le = m <= ORDER ? nw[m] : 0.;    // sh <= 0
+        const double s1_gt = m >= 1 ? nw[m - 1] : 0.;    // sh == +1
+        const double a = w0 == 0 ? s0_w0 : s0_w1;
+        const double b = sh <= 0 ? s1_le : s1_gt;
+        S0w[m] = a;
+        DSw[m] = b - a;
+    }
+}
+
+// Windowed walk of the 3D Esirkepov contributions (same values as
+// esirkepov3d for every point it visits; the points it skips receive exact
+// zeros there).  add(component, i, j, k, value) gets reference stencil
+// indices (0..ORDER+2) along the three axes.
+template <int ORDER, class Add>
+__device__ inline void esirkepov3d_win(const double (&S0)[3][ORDER + 2], const double (&DS)[3][ORDER + 2],
+                                       const int *w0, const double *cr, Add add)
+{
+    constexpr int W = ORDER + 2, S = ORDER + 3;
+    // Jx: columns (y, z) of the window, prefix along x
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const double Wt = weight3(S0[1], DS[1], S0[2], DS[2], a, b);
+            const double t = cr[0] * Wt;
+            double acc = 0.;
+#pragma unroll
+            for (int r = 1; r <= W; ++r) {
+                if constexpr (ORDER == 2) acc -= DS[0][r - 1] * t;
+                else acc -= cr[0] * DS[0][r - 1] * Wt;
+                if (r < W || w0[0] == 0) add(0, w0[0] + r, w0[1] + a, w0[2] + b, acc);
+            }
+        }
+    // Jy: columns (x, z), prefix along y
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const double Wt = weight3(S0[2], DS[2], S0[0], DS[0], b, a);
+            const double t = cr[1] * Wt;
+            double acc = 0.;
+#pragma unroll
+            for (int r = 1; r <= W; ++r) {
+                if constexpr (ORDER == 2) acc -= DS[1][r - 1] * t;
+                else acc -= cr[1] * DS[1][r - 1] * Wt;
+                if (r < W || w0[1] == 0) add(1, w0[0] + a, w0[1] + r, w0[2] + b, acc);
+            }
+        }
+    // Jz: columns (x, y), prefix along z
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const double Wt = weight3(S0[0], DS[0], S0[1], DS[1], a, b);
+            const double t = cr[2] * Wt;
+            double acc = 0.;
+#pragma unroll
+            for (int r = 1; r <= W; ++r) {
+                if constexpr (ORDER == 2) acc -= DS[2][r - 1] * t;
+                else acc -= cr[2] * DS[2][r - 1] * Wt;
+                if (r < W || w0[2] == 0) add(2, w0[0] + a, w0[1] + b, w0[2] + r, acc);
+            }
+        }
+    (void)S;
+}
+
 // 2D3V Esirkepov of Projector2D2Order::currents (Projector2D2Order.cpp:
 // 131-188): Jx prefix along x per column j, Jy prefix along y per row i,
 // Jz projected with the time-averaged shape.  add(c, i, j, v).

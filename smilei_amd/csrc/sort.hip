@@ -122,14 +122,30 @@ k_scan_apply(const int32_t *__restrict__ in, int64_t n, const int32_t *__restric
     }
 }
 
+// Scatter of slot indices into their cell buckets.  Consecutive slots mostly
+// keep their cell, so lanes of a wavefront with equal keys form runs: the
+// run head takes `run` places with one atomic and the lanes fill them in
+// slot order.  The count-down returns the histogram to zero.
 __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
                           int32_t *__restrict__ counts, int32_t *__restrict__ perm)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t g = keys[i] & kKeyMask;
-    const int c = atomicSub(&counts[g], 1);
-    perm[cell_off[g] + c - 1] = (int32_t)i;
+    const int lane = threadIdx.x & 63;
+    const uint32_t g = i < n ? keys[i] & kKeyMask : 0xFFFFFFFFu;
+    const uint32_t gp = __shfl_up(g, 1, 64);
+    const bool head = lane == 0 || gp != g;
+    const uint64_t heads = __ballot(head);
+    const uint64_t above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
+    const int my_head = 63 - __clzll((unsigned long long)(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1))));
+    int c = 0;
+    if (head && g != 0xFFFFFFFFu) c = atomicSub(&counts[g], next - lane);
+    const int ch = __shfl(c, my_head, 64);
+    const int run_next = __shfl(next, my_head, 64);
+    if (g == 0xFFFFFFFFu) return;
+    // the run occupies places [c - run, c) counted from the bucket start
+    const int run = run_next - my_head;
+    perm[cell_off[g] + ch - run + (lane - my_head)] = (int32_t)i;
 }
 
 // Canonical order inside a cell: by direction of motion, then source slot.
@@ -191,10 +207,10 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 int64_t sort_bsum_size(int64_t ncells) { return (ncells + kScanTile - 1) / kScanTile + 1; }
 
-void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st)
+void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st, bool counted)
 {
     const int64_t nc = g.ncells;
-    if (n > 0) {
+    if (n > 0 && !counted) {
         hipLaunchKernelGGL(k_hist, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, sb.counts);
         SMI_LAUNCH_CHECK();
     }
