@@ -30,7 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_PUSH = {"f64": 108.0}
+# algorithmic particle bytes per push (read + write of the particle state)
+BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 
 
 def cfl_dt(dx, factor=0.95):
@@ -44,10 +45,11 @@ def workload():
                 species=[("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))])
 
 
-def run_gpu(args, W):
+def run_gpu(args, W, form):
     import smilei_amd as sa
+    prec = sa.MIXED if form == "mixed" else sa.PLAIN_FP64
     sim = sa.Simulation("3Dcartesian", 2, W["shape"], W["patch"], W["dx"], W["dt"],
-                        precision=sa.PLAIN_FP64, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
+                        precision=prec, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
     ncell = int(np.prod(W["shape"]))
     for k, (name, m, q, sig) in enumerate(W["species"]):
         sid = sim.add_species(name, m, q, ncell * W["ppc"])
@@ -112,47 +114,61 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="mixed,fp64", help="comma list of mixed,fp64; the first is the headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the PIC step is not sharded across GPUs here); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     W = workload()
-    g = run_gpu(args, W)
-    npart, K = g["npart"], args.steps
-    value = npart * K / g["wall"]
-    t_part = g["kernel_ms"]["particles"] / 1e3 / K  # s per step, both species
-    achieved = BYTES_PER_PUSH["f64"] * npart / t_part / 1e9
+    forms = [f for f in args.forms.split(",") if f]
+    res = {}
+    for form in forms:
+        g = run_gpu(args, W, form)
+        npart, K = g["npart"], args.steps
+        t_part = g["kernel_ms"]["particles"] / 1e3 / K  # s per step, both species
+        achieved = BYTES_PER_PUSH[form] * npart / t_part / 1e9
+        res[form] = dict(value=npart * K / g["wall"], ms_per_step=1e3 * g["wall"] / K,
+                         ms_per_step_graph=1e3 * g["wall_graph"] / K,
+                         kernel_ms_per_step={k: v / K for k, v in g["kernel_ms"].items()},
+                         achieved=achieved, npart=npart, energies=g["energies"])
+    head = forms[0]
+    h = res[head]
     out = {
         "metric": "particle-pushes/s (gather+Boris+Esirkepov) 1xMI355X, mixed vs FP64; %HBM peak",
-        "value": value,
+        "value": h["value"],
         "unit": "particle-pushes/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * g["wall"] / K,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if head == "mixed" else "f64",
         "data": "synthetic (seeded thermal plasma initialised in HBM)",
         "config": {
             "workload": "3D thermal plasma 256x128x128 cells, dx=0.2, 16^3-cell patches, e-/p+ 8 ppc each "
-                        "(%d particles), order 2, Yee, periodic, plain FP64 form, atomic deposit" % npart,
-            "particles": npart,
-            "ms_per_step_graph": 1e3 * g["wall_graph"] / K,
-            "kernel_ms_per_step": {k: v / K for k, v in g["kernel_ms"].items()},
+                        "(%d particles), order 2, Yee, periodic, atomic tile deposit" % h["npart"],
+            "form": head,
+            "particles": h["npart"],
+            "forms": {f: {k: v for k, v in r.items() if k != "energies"} for f, r in res.items()},
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": h["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": h["achieved"] / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_particles<3,2,deposit> (fused gather+Boris+key+Esirkepov), 108 B/push algorithmic",
+            "kernel": "%s particle kernel (fused gather+Boris+key+Esirkepov), %d B/push algorithmic"
+                      % (head, BYTES_PER_PUSH[head]),
         },
     }
+    if "mixed" in res and "fp64" in res:
+        out["config"]["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+        em, ef = np.array(res["mixed"]["energies"]), np.array(res["fp64"]["energies"])
+        out["config"]["rel_energy_diff_mixed_vs_fp64"] = float(abs(em.sum() - ef.sum()) / abs(ef.sum()))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(args, W)
     print(json.dumps(out))

@@ -135,6 +135,12 @@ int smi_gauss(smi_context *ctx, double *global);
 int smi_profile_step(smi_context *ctx, float *ms);
 /* The stream all work of the context is enqueued on (hipStream_t). */
 void *smi_stream(smi_context *ctx);
+/* Launch-shape knobs searched by the autotuner (no effect on results beyond
+ * the order of the atomic J summation).  SMI_TUNE_INTERLEAVE: 1 = lane l of
+ * a wavefront takes the l-th slice of its cluster's cell-sorted particles
+ * (lanes in distinct cells), 0 = consecutive particles. */
+#define SMI_TUNE_INTERLEAVE 1
+int smi_set_tuning(smi_context *ctx, int key, int value);
 
 /* ---- per-operator entry points (one patch, reference layout) -------- *
  * Field arrays are the patch's ghosted arrays: primal extent

@@ -57,6 +57,7 @@ SIGNATURES = {
     "smi_gauss": (ctypes.c_int, [_VP, _D]),
     "smi_profile_step": (ctypes.c_int, [_VP, _F]),
     "smi_stream": (_VP, [_VP]),
+    "smi_set_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "smi_interpolate_3d": (ctypes.c_int, [ctypes.c_int] + [_VP] * 6 + [_I3, _I3, _D3, ctypes.c_int64] + [_VP] * 3
                            + [_VP] * 4 + [_VP]),
     "smi_interpolate_2d": (ctypes.c_int, [ctypes.c_int] + [_VP] * 6 + [ctypes.c_int * 2, ctypes.c_int * 2,

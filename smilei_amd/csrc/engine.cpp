@@ -34,15 +34,22 @@ struct SpeciesState {
     std::string name;
     double mass = 1, charge = -1;
     int64_t cap = 0, n = 0;
-    double *buf[2][7] = {};
-    uint32_t *keys = nullptr;
+    void *buf[2][7] = {};          // x y z ux uy uz w: FP64, or FP32 (offsets) in the mixed form
+    uint32_t *keys[2] = {};        // cell keys of the particles of buffer b
     SortBufs sb{};
-    double *scratch = nullptr;  // 7*cap, deterministic mode
+    double *scratch = nullptr;     // 7*cap, deterministic mode
     SpeciesK k{};
+    SpeciesKM km{};
 
     PartPtrs view(int b) const
     {
-        return PartPtrs{buf[b][0], buf[b][1], buf[b][2], buf[b][3], buf[b][4], buf[b][5], buf[b][6]};
+        double *const *q = (double *const *)buf[b];
+        return PartPtrs{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
+    }
+    MixPtrs mview(int b) const
+    {
+        float *const *q = (float *const *)buf[b];
+        return MixPtrs{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
     }
 };
 
@@ -72,6 +79,8 @@ struct smi_context {
     double *rho = nullptr;         // diagnostics scratch (ncells)
     int *err = nullptr;            // device flag raised by kernels on corrupt particles
     int cur = 0;                   // particle buffer holding the current state
+    bool mixed = false;            // SMI_MIXED form
+    int interleave = 1;            // lane-to-particle mapping of the particle kernel (tuning)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev[6] = {};
 
@@ -116,8 +125,8 @@ void upload_det(smi_context *c)
             d.sp = s.k;
             d.cell_off = s.sb.cell_off;
             d.scratch = s.scratch;
-            d.pz = s.buf[1 - k][5];
-            d.w = s.buf[1 - k][6];
+            d.pz = (const double *)s.buf[1 - k][5];
+            d.w = (const double *)s.buf[1 - k][6];
             d.invgf = nullptr;
             d.cap = s.cap;
             v.push_back(d);
@@ -138,8 +147,12 @@ void enqueue_step(smi_context *c, bool timed)
     if (!det)
         for (int q = 0; q < 3; ++q) launch_fill(f.J[q], g.ncells, 0.0, c->st);  // restartRhoJ
     for (auto &s : c->sp) {
-        launch_particles(g, s.k, s.view(a), s.sb.perm, s.view(1 - a), s.keys, s.scratch, s.cap, s.sb.cell_off, GF, J,
-                         !det, c->err, s.sb.counts, c->st);
+        if (c->mixed)
+            launch_particles_mixed(g, s.km, s.mview(a), s.keys[a], s.sb.perm, s.mview(1 - a), s.keys[1 - a],
+                                   s.sb.cell_off, GF, J, c->err, s.sb.counts, c->interleave != 0, c->st);
+        else
+            launch_particles(g, s.k, s.view(a), s.sb.perm, s.view(1 - a), s.keys[1 - a], s.scratch, s.cap,
+                             s.sb.cell_off, GF, J, !det, c->err, s.sb.counts, c->st);
     }
     if (timed) SMI_HIP(hipEventRecord(c->ev[1], c->st));
     if (det && !c->sp.empty()) {
@@ -149,7 +162,7 @@ void enqueue_step(smi_context *c, bool timed)
     if (timed) SMI_HIP(hipEventRecord(c->ev[2], c->st));
     launch_maxwell(g, f, c->st);
     if (timed) SMI_HIP(hipEventRecord(c->ev[3], c->st));
-    for (auto &s : c->sp) launch_sort(g, s.keys, s.n, s.sb, c->st, /*counted=*/true);
+    for (auto &s : c->sp) launch_sort(g, s.keys[1 - a], s.n, s.sb, c->st, /*counted=*/true);
     if (timed) SMI_HIP(hipEventRecord(c->ev[4], c->st));
 }
 
@@ -195,7 +208,10 @@ int smi_create(const smi_params *p, smi_context **out)
         throw Error(SMI_ERR_INVALID, std::string("unsupported maxwell_solver: ") + p->maxwell_solver);
     if (p->precision != SMI_PLAIN_FP64 && p->precision != SMI_MIXED)
         throw Error(SMI_ERR_INVALID, "precision must be SMI_PLAIN_FP64 or SMI_MIXED");
-    if (p->precision == SMI_MIXED) throw Error(SMI_ERR_INVALID, "mixed precision form not available in this build");
+    if (p->precision == SMI_MIXED && nd != 3)
+        throw Error(SMI_ERR_INVALID, "the mixed-precision form is available in 3Dcartesian");
+    if (p->precision == SMI_MIXED && p->deposit == SMI_DEPOSIT_DETERMINISTIC)
+        throw Error(SMI_ERR_INVALID, "the deterministic deposit is a mode of the plain FP64 form");
     if (p->deposit != SMI_DEPOSIT_ATOMIC && p->deposit != SMI_DEPOSIT_DETERMINISTIC)
         throw Error(SMI_ERR_INVALID, "deposit must be SMI_DEPOSIT_ATOMIC or SMI_DEPOSIT_DETERMINISTIC");
     if (!(p->timestep > 0)) throw Error(SMI_ERR_INVALID, "timestep must be > 0");
@@ -241,6 +257,7 @@ int smi_create(const smi_params *p, smi_context **out)
     auto *c = new smi_context();
     c->p = *p;
     c->g = g;
+    c->mixed = p->precision == SMI_MIXED;
     try {
         SMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
         for (int f = 0; f < SMI_NFIELDS; ++f) {
@@ -272,7 +289,7 @@ int smi_destroy(smi_context *c)
     for (auto &s : c->sp) {
         for (auto &bb : s.buf)
             for (auto &q : bb) (void)hipFree(q);
-        (void)hipFree(s.keys);
+        for (auto &k : s.keys) (void)hipFree(k);
         (void)hipFree(s.sb.counts);
         (void)hipFree(s.sb.cell_off);
         (void)hipFree(s.sb.perm);
@@ -305,8 +322,8 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.charge = charge;
     s.cap = capacity;
     for (auto &bb : s.buf)
-        for (auto &q : bb) q = dalloc<double>(capacity);
-    s.keys = dalloc<uint32_t>(capacity);
+        for (auto &q : bb) q = c->mixed ? (void *)dalloc<float>(capacity) : (void *)dalloc<double>(capacity);
+    for (auto &k : s.keys) k = dalloc<uint32_t>(capacity);
     s.sb.counts = dalloc<int32_t>(c->g.ncells);
     s.sb.cell_off = dalloc<int32_t>(c->g.ncells + 1);
     s.sb.perm = dalloc<int32_t>(capacity);
@@ -323,6 +340,9 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.k.mass = mass;
     s.k.qmh = charge * one_over_mass * dts2;
     s.k.cw_scale = (1. / cell_volume) * charge;
+    s.km.qmh = (float)s.k.qmh;
+    s.km.cw_scale = (float)s.k.cw_scale;
+    for (int d = 0; d < 3; ++d) s.km.dxdt[d] = (float)(c->g.dx[d] / c->g.dt);
     c->sp.push_back(s);
     // deterministic-deposit descriptors for both ping-pong parities
     (void)hipFree(c->d_det);
@@ -354,17 +374,32 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
     SMI_HIP(hipSetDevice(c->p.device));
     drop_graphs(c);
     SMI_HIP(hipStreamSynchronize(c->st));
-    for (int q = 0; q < 7; ++q) {
-        if (q == 2 && c->g.nd < 3) {
-            launch_fill(S.buf[c->cur][q], n, 0.0, c->st);
-            continue;
-        }
-        if (!src[q]) throw Error(SMI_ERR_INVALID, "null particle array");
-        if (n) SMI_HIP(hipMemcpyAsync(S.buf[c->cur][q], src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
-    }
     S.n = n;
-    launch_bin_initial(c->g, S.view(c->cur), n, S.keys, c->st);
-    launch_sort(c->g, S.keys, n, S.sb, c->st);
+    if (c->mixed) {
+        // stage FP64 on the device, convert to cell + FP32 offset
+        std::vector<double *> tmp(7, nullptr);
+        for (int q = 0; q < 7; ++q) {
+            tmp[q] = dalloc<double>(n);
+            if (!src[q]) throw Error(SMI_ERR_INVALID, "null particle array");
+            if (n) SMI_HIP(hipMemcpyAsync(tmp[q], src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        }
+        launch_to_mixed(c->g, PartPtrs{tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]}, n, S.mview(c->cur),
+                        S.keys[c->cur], c->st);
+        SMI_HIP(hipStreamSynchronize(c->st));
+        for (auto t : tmp) (void)hipFree(t);
+    } else {
+        for (int q = 0; q < 7; ++q) {
+            double *dst = (double *)S.buf[c->cur][q];
+            if (q == 2 && c->g.nd < 3) {
+                launch_fill(dst, n, 0.0, c->st);
+                continue;
+            }
+            if (!src[q]) throw Error(SMI_ERR_INVALID, "null particle array");
+            if (n) SMI_HIP(hipMemcpyAsync(dst, src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        }
+        launch_bin_initial(c->g, S.view(c->cur), n, S.keys[c->cur], c->st);
+    }
+    launch_sort(c->g, S.keys[c->cur], n, S.sb, c->st);
     SMI_HIP(hipStreamSynchronize(c->st));
     return SMI_OK;
     SMI_CATCH
@@ -405,11 +440,12 @@ int smi_init_species(smi_context *c, int s, int ppc, double density, double sigm
     }
     double cell_volume = 1.0;
     for (int d = 0; d < g.nd; ++d) cell_volume *= g.dx[d];
-    launch_init(g, S.view(c->cur), n, ppc, d_cells, density * cell_volume / ppc, sigma, drift_ux,
-                seed * 0x100000001B3ull + (uint64_t)s, c->st);
+    const MixPtrs none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    launch_init(g, S.view(c->cur), c->mixed ? S.mview(c->cur) : none, S.keys[c->cur], n, ppc, d_cells,
+                density * cell_volume / ppc, sigma, drift_ux, seed * 0x100000001B3ull + (uint64_t)s, c->st);
     S.n = n;
-    launch_bin_initial(g, S.view(c->cur), n, S.keys, c->st);
-    launch_sort(g, S.keys, n, S.sb, c->st);
+    if (!c->mixed) launch_bin_initial(g, S.view(c->cur), n, S.keys[c->cur], c->st);
+    launch_sort(g, S.keys[c->cur], n, S.sb, c->st);
     SMI_HIP(hipStreamSynchronize(c->st));
     (void)hipFree(d_cells);
     return SMI_OK;
@@ -437,17 +473,29 @@ int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, do
     std::vector<double> tmp(n);
     if (n) {
         SMI_HIP(hipMemcpy(perm.data(), S.sb.perm, n * 4, hipMemcpyDeviceToHost));
-        SMI_HIP(hipMemcpy(keys.data(), S.keys, n * 4, hipMemcpyDeviceToHost));
+        SMI_HIP(hipMemcpy(keys.data(), S.keys[c->cur], n * 4, hipMemcpyDeviceToHost));
     }
     double *dst[7] = {x, y, z, px, py, pz, w};
-    for (int q = 0; q < 7; ++q) {
-        if (!dst[q]) continue;
-        if (q == 2 && c->g.nd < 3) {
-            std::fill(dst[q], dst[q] + n, 0.0);
-            continue;
+    if (c->mixed) {
+        // FP64 global positions in sorted order, built on the device
+        std::vector<double *> t(7, nullptr);
+        for (auto &q : t) q = dalloc<double>(n);
+        launch_from_mixed(c->g, S.mview(c->cur), S.keys[c->cur], S.sb.perm, n,
+                          PartPtrs{t[0], t[1], t[2], t[3], t[4], t[5], t[6]}, c->st);
+        SMI_HIP(hipStreamSynchronize(c->st));
+        for (int q = 0; q < 7; ++q)
+            if (dst[q] && n) SMI_HIP(hipMemcpy(dst[q], t[q], n * sizeof(double), hipMemcpyDeviceToHost));
+        for (auto q : t) (void)hipFree(q);
+    } else {
+        for (int q = 0; q < 7; ++q) {
+            if (!dst[q]) continue;
+            if (q == 2 && c->g.nd < 3) {
+                std::fill(dst[q], dst[q] + n, 0.0);
+                continue;
+            }
+            if (n) SMI_HIP(hipMemcpy(tmp.data(), S.buf[c->cur][q], n * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < n; ++j) dst[q][j] = tmp[perm[j]];
         }
-        if (n) SMI_HIP(hipMemcpy(tmp.data(), S.buf[c->cur][q], n * sizeof(double), hipMemcpyDeviceToHost));
-        for (int64_t j = 0; j < n; ++j) dst[q][j] = tmp[perm[j]];
     }
     const uint32_t mask = (1u << 27) - 1;
     for (int64_t j = 0; j < n; ++j) {
@@ -561,6 +609,20 @@ int smi_profile_step(smi_context *c, float *ms)
 
 void *smi_stream(smi_context *c) { return c ? (void *)c->st : nullptr; }
 
+int smi_set_tuning(smi_context *c, int key, int value)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (key == SMI_TUNE_INTERLEAVE) {
+        c->interleave = value != 0;
+        drop_graphs(c);
+    } else {
+        throw Error(SMI_ERR_INVALID, "unknown tuning key");
+    }
+    return SMI_OK;
+    SMI_CATCH
+}
+
 int smi_energies(smi_context *c, double *out)
 {
     SMI_TRY
@@ -578,7 +640,8 @@ int smi_energies(smi_context *c, double *out)
     out[0] = 0.5 * cell_volume * s;
     for (size_t k = 0; k < c->sp.size(); ++k) {
         SpeciesState &S = c->sp[k];
-        launch_ukin(S.view(c->cur), S.sb.perm, S.n, c->partials, kReduceBlocks, c->st);
+        if (c->mixed) launch_ukin_mixed(S.mview(c->cur), S.sb.perm, S.n, c->partials, kReduceBlocks, c->st);
+        else launch_ukin(S.view(c->cur), S.sb.perm, S.n, c->partials, kReduceBlocks, c->st);
         SMI_HIP(hipMemcpyAsync(h.data(), c->partials, kReduceBlocks * sizeof(double), hipMemcpyDeviceToHost, c->st));
         SMI_HIP(hipStreamSynchronize(c->st));
         double e = 0;
@@ -592,7 +655,10 @@ int smi_energies(smi_context *c, double *out)
 static void compute_rho(smi_context *c)
 {
     launch_fill(c->rho, c->g.ncells, 0.0, c->st);
-    for (auto &S : c->sp) launch_rho(c->g, S.k, S.view(c->cur), S.sb.perm, S.n, c->rho, c->st);
+    for (auto &S : c->sp) {
+        if (c->mixed) launch_rho_mixed(c->g, S.km, S.mview(c->cur), S.keys[c->cur], S.sb.perm, S.n, c->rho, c->st);
+        else launch_rho(c->g, S.k, S.view(c->cur), S.sb.perm, S.n, c->rho, c->st);
+    }
 }
 
 int smi_rho(smi_context *c, double *gl)

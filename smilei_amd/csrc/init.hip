@@ -27,8 +27,8 @@ __device__ inline double u01(uint64_t seed, uint64_t i, uint32_t k)
     return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-__global__ void k_init(Geom g, PartPtrs A, int64_t n, int ppc, const int32_t *__restrict__ cells, double weight,
-                       double sigma, double drift, uint64_t seed)
+__global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ keys, int64_t n, int ppc,
+                       const int32_t *__restrict__ cells, double weight, double sigma, double drift, uint64_t seed)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -37,9 +37,10 @@ __global__ void k_init(Geom g, PartPtrs A, int64_t n, int ppc, const int32_t *__
     const int cj = (int)((c / g.N[2]) % g.N[1]);
     const int ci = (int)(c / ((int64_t)g.N[2] * g.N[1]));
     const int cc[3] = {ci, cj, ck};
-    double x[3];
+    double x[3], f[3];
     for (int d = 0; d < 3; ++d) {
-        double v = ((double)cc[d] + u01(seed, i, d)) * g.dx[d];
+        f[d] = d < g.nd ? u01(seed, i, d) : 0.;
+        double v = ((double)cc[d] + f[d]) * g.dx[d];
         if (v >= g.xmax[d]) v = g.xmax[d] - 0.5 * g.dx[d] * 1e-9;  // guard rounding at the box end
         x[d] = v;
     }
@@ -47,6 +48,30 @@ __global__ void k_init(Geom g, PartPtrs A, int64_t n, int ppc, const int32_t *__
     for (int d = 0; d < 3; ++d) {
         const double r1 = u01(seed, i, 3 + 2 * d), r2 = u01(seed, i, 4 + 2 * d);
         u[d] = sigma * sqrt(-2.0 * log(r1)) * cos(6.283185307179586 * r2);
+    }
+    if (M.ox) {
+        // split-precision form: exact cell, FP32 offset of the same draw
+        int pc[3], l[3];
+        for (int d = 0; d < 3; ++d) {
+            pc[d] = cc[d] / g.n[d];
+            l[d] = cc[d] - pc[d] * g.n[d];
+        }
+        const int p = (pc[0] * g.npatch[1] + pc[1]) * g.npatch[2] + pc[2];
+        const int cl = ((l[0] >> 2) * g.ncl[1] + (l[1] >> 2)) * g.ncl[2] + (l[2] >> 2);
+        keys[i] = (13u << 27) | (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
+        float fo[3];
+        for (int d = 0; d < 3; ++d) {
+            fo[d] = (float)f[d];
+            if (fo[d] >= 1.f) fo[d] = 0.99999994f;
+        }
+        M.ox[i] = fo[0];
+        M.oy[i] = fo[1];
+        M.oz[i] = fo[2];
+        M.ux[i] = (float)(u[0] + drift);
+        M.uy[i] = (float)u[1];
+        M.uz[i] = (float)u[2];
+        M.w[i] = (float)weight;
+        return;
     }
     A.x[i] = x[0];
     A.y[i] = x[1];
@@ -59,12 +84,12 @@ __global__ void k_init(Geom g, PartPtrs A, int64_t n, int ppc, const int32_t *__
 
 }  // namespace
 
-void launch_init(const Geom &g, const PartPtrs &A, int64_t n, int ppc, const int32_t *cells, double weight,
-                 double sigma, double drift, uint64_t seed, hipStream_t st)
+void launch_init(const Geom &g, const PartPtrs &A, const MixPtrs &M, uint32_t *keys, int64_t n, int ppc,
+                 const int32_t *cells, double weight, double sigma, double drift, uint64_t seed, hipStream_t st)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, A, n, ppc, cells, weight, sigma,
-                       drift, seed);
+    hipLaunchKernelGGL(k_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, A, M, keys, n, ppc, cells,
+                       weight, sigma, drift, seed);
     SMI_LAUNCH_CHECK();
 }
 

@@ -12,6 +12,19 @@ struct PartPtrs {
     double *x, *y, *z, *px, *py, *pz, *w;
 };
 
+// Mixed-precision particle storage: FP32 offsets inside the cell (the cell
+// itself is the key), FP32 momenta and weight.
+struct MixPtrs {
+    float *ox, *oy, *oz, *ux, *uy, *uz, *w;
+};
+
+// Species constants of the mixed kernel (FP32).
+struct SpeciesKM {
+    float qmh;       // charge/mass * dt/2
+    float cw_scale;  // charge / V_cell
+    float dxdt[3];   // dx/dt, dy/dt, dz/dt
+};
+
 // Fields read by the gather (E and the centred B_m).
 struct GatherFields {
     const double *Ex, *Ey, *Ez, *Bx, *By, *Bz;
@@ -70,10 +83,27 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
 // Keys of freshly loaded particles (positions in A, slot order = input).
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
 
+// ---- mixed-precision form (particles_mixed.hip) -------------------------
+// Same contract as launch_particles (always the atomic tile deposit); the
+// cell of particle i is keysA[i], its new key goes to keysB[slot].
+void launch_particles_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &A, const uint32_t *keysA,
+                            const int32_t *perm, const MixPtrs &B, uint32_t *keysB, const int32_t *cell_off,
+                            const GatherFields &F, const JPtrs &J, int *err, int32_t *counts, bool interleave,
+                            hipStream_t st);
+void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
+void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm, int64_t n,
+                       const PartPtrs &X, hipStream_t st);
+void launch_ukin_mixed(const MixPtrs &M, const int32_t *perm, int64_t n, double *partials, int nblocks,
+                       hipStream_t st);
+void launch_rho_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,
+                      int64_t n, double *rho, hipStream_t st);
+
 // ---- initialisation (init.hip) ---------------------------------------
 // Particle i lives in cell cells[i/ppc] (or cell i/ppc when cells == null).
-void launch_init(const Geom &g, const PartPtrs &A, int64_t n, int ppc, const int32_t *cells, double weight,
-                 double sigma, double drift, uint64_t seed, hipStream_t st);
+// Writes FP64 particles to A, or (when M.ox != null) mixed particles to M
+// plus their cell keys (direction 13) to `keys`.
+void launch_init(const Geom &g, const PartPtrs &A, const MixPtrs &M, uint32_t *keys, int64_t n, int ppc,
+                 const int32_t *cells, double weight, double sigma, double drift, uint64_t seed, hipStream_t st);
 
 // ---- fields (fields.hip) ----------------------------------------------
 struct FieldSet {

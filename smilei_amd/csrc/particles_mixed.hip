@@ -1,0 +1,510 @@
+// particles_mixed.hip -- the mixed-precision form of the particle step.
+//
+// Storage per particle (split precision): the cell is an exact int32 (the
+// in-patch cell key, which is also the sort key, kept in `keys`), the
+// position inside the cell an FP32 offset in [0,1) per axis, momenta u in
+// FP32, weight FP32: 32 B read, 28 B written per push (+4 B weight copy
+// of the out-of-place ping-pong).
+//
+// Arithmetic: shape factors, the gather, the Boris rotation and the
+// Esirkepov contributions in FP32 with fused multiply-adds; the gathered
+// E/B are staged from the FP64 field arrays; each cluster's J tile is
+// accumulated in FP32 in LDS and flushed with FP64 atomics into the FP64 J
+// arrays (the global J reduction stays FP64, as do E, B and the Yee
+// update).  The cell change and the periodic wrap are exact integer
+// operations, so no rounding can misplace a particle at a patch face.
+//
+// Same call chain as the FP64 kernel: Interpolator3D{2,4}Order (gather),
+// PusherBoris, PartBoundCond (direction of motion), Cluster3D::CellKey,
+// Projector3D{2,4}Order::currents (Esirkepov).
+#include "kernels.h"
+#include "pic_device.h"
+
+namespace smi {
+
+namespace {
+
+using namespace dev;
+
+constexpr uint32_t kMask = (1u << 27) - 1;
+
+// Quadratic / quartic B-spline weights at offsets -H..H, FP32.
+template <int ORDER>
+__device__ inline void splinef(float d, float *w)
+{
+    if constexpr (ORDER == 2) {
+        w[0] = 0.5f * fmaf(d, d - 1.f, 0.25f);
+        w[1] = fmaf(-d, d, 0.75f);
+        w[2] = 0.5f * fmaf(d, d + 1.f, 0.25f);
+    } else {
+        const float d2 = d * d;
+        // 1/384 (1-2d)^4 and friends, Horner form of Interpolator3D4Order.h:62-67
+        const float a = 0.5f - d, b = 0.5f + d;
+        const float a2 = a * a, b2 = b * b;
+        w[0] = a2 * a2 * (1.f / 24.f);
+        w[4] = b2 * b2 * (1.f / 24.f);
+        w[2] = fmaf(d2, fmaf(d2, 0.25f, -0.625f), 115.f / 192.f);
+        const float c = fmaf(d2, -(1.f / 6.f), 0.25f);
+        w[1] = fmaf(d, fmaf(d, c, 1.f / 6.f * d2 - 11.f / 24.f), 19.f / 96.f);
+        w[3] = fmaf(d, fmaf(d, c, -(1.f / 6.f) * d2 + 11.f / 24.f), 19.f / 96.f);
+    }
+}
+
+// Gather Sum_{a,b,c} X[a] Y[b] Z[c] f(i+a, j+b, k+c) from an LDS tile of
+// width T, contracted z first, then y, then x (3 FMA chains).
+template <int H, int T>
+__device__ inline float gatherf(const float *X, const float *Y, const float *Z, const float *t, int i, int j, int k)
+{
+    float r = 0.f;
+#pragma unroll
+    for (int a = -H; a <= H; ++a) {
+        float ta = 0.f;
+#pragma unroll
+        for (int b = -H; b <= H; ++b) {
+            const float *row = t + ((i + a) * T + (j + b)) * T + k;
+            float sb = 0.f;
+#pragma unroll
+            for (int c = -H; c <= H; ++c) sb = fmaf(Z[c + H], row[c], sb);
+            ta = fmaf(Y[b + H], sb, ta);
+        }
+        r = fmaf(X[a + H], ta, r);
+    }
+    return r;
+}
+
+// Esirkepov window in FP32 (see esirkepov_window): S0 from the old delta,
+// S1 from the new delta shifted by sh in {-1,0,1}.
+template <int ORDER>
+__device__ inline void windowf(float dold, float dnew, int sh, float *S0w, float *DSw, int &w0)
+{
+    constexpr int W = ORDER + 2;
+    float s[ORDER + 1], nw[ORDER + 1];
+    splinef<ORDER>(dold, s);
+    splinef<ORDER>(dnew, nw);
+    w0 = sh < 0 ? 0 : 1;
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        const float a = w0 == 0 ? (m == 0 ? 0.f : s[m - 1]) : (m <= ORDER ? s[m] : 0.f);
+        const float b = sh <= 0 ? (m <= ORDER ? nw[m] : 0.f) : (m >= 1 ? nw[m - 1] : 0.f);
+        S0w[m] = a;
+        DSw[m] = b - a;
+    }
+}
+
+// Windowed 3D Esirkepov with W(b,c) = S0b (S0c + DSc/2) + DSb (S0c/2 + DSc/3)
+// (the reference's four-term weight, factorised), FP32.
+template <int ORDER, class Add>
+__device__ inline void esirkepov3f(const float (&S0)[3][ORDER + 2], const float (&DS)[3][ORDER + 2], const int *w0,
+                                   const float *cr, Add add)
+{
+    constexpr int W = ORDER + 2;
+    float P[3][W], Q[3][W];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < W; ++m) {
+            P[d][m] = fmaf(0.5f, DS[d][m], S0[d][m]);
+            Q[d][m] = fmaf(1.f / 3.f, DS[d][m], 0.5f * S0[d][m]);
+        }
+    // component c along axis c, transverse axes (b, e) cyclic
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int bx = (c + 1) % 3, ex = (c + 2) % 3;
+#pragma unroll
+        for (int a = 0; a < W; ++a)
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                const float Wt = fmaf(S0[bx][a], P[ex][b], DS[bx][a] * Q[ex][b]);
+                const float t = cr[c] * Wt;
+                float acc = 0.f;
+#pragma unroll
+                for (int r = 1; r <= W; ++r) {
+                    acc = fmaf(-DS[c][r - 1], t, acc);
+                    if (r < W || w0[c] == 0) {
+                        int idx[3];
+                        idx[c] = w0[c] + r;
+                        idx[bx] = w0[bx] + a;
+                        idx[ex] = w0[ex] + b;
+                        add(c, idx[0], idx[1], idx[2], acc);
+                    }
+                }
+            }
+    }
+}
+
+struct Cl {
+    int p, pc[3], G0[3];
+    int64_t gbase;
+};
+
+__device__ inline Cl cluster_of(const Geom &g, int b)
+{
+    Cl c;
+    c.p = b / g.clpp;
+    const int cl = b % g.clpp;
+    const int cc[3] = {cl / (g.ncl[1] * g.ncl[2]), (cl / g.ncl[2]) % g.ncl[1], cl % g.ncl[2]};
+    c.pc[0] = c.p / (g.npatch[1] * g.npatch[2]);
+    c.pc[1] = (c.p / g.npatch[2]) % g.npatch[1];
+    c.pc[2] = c.p % g.npatch[2];
+    for (int d = 0; d < 3; ++d) c.G0[d] = c.pc[d] * g.n[d] + 4 * cc[d];
+    c.gbase = (int64_t)c.p * g.cpp + (int64_t)cl * g.cells_per_cl;
+    return c;
+}
+
+// Global cell key of global cell (cx,cy,cz) (already wrapped).
+__device__ inline uint32_t global_key3(const Geom &g, const int *c, int *pc)
+{
+    int l[3];
+    for (int d = 0; d < 3; ++d) {
+        pc[d] = c[d] / g.n[d];
+        l[d] = c[d] - pc[d] * g.n[d];
+    }
+    const int p = (pc[0] * g.npatch[1] + pc[1]) * g.npatch[2] + pc[2];
+    const int cl = ((l[0] >> 2) * g.ncl[1] + (l[1] >> 2)) * g.ncl[2] + (l[2] >> 2);
+    return (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
+}
+
+template <int ORDER, bool INTERLEAVE>
+__global__ void __launch_bounds__(256)
+k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ keysA, const int32_t *__restrict__ perm,
+                  MixPtrs B, uint32_t *__restrict__ keysB, const int32_t *__restrict__ cell_off, GatherFields F,
+                  JPtrs J, int *__restrict__ err, int32_t *__restrict__ counts)
+{
+    constexpr int H = ORDER / 2;
+    constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
+    constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
+    __shared__ float sF[6 * GT3];
+    __shared__ float sJ[3 * JT3];
+    __shared__ int sHist[64];
+
+    const Cl cl = cluster_of(g, blockIdx.x);
+    const int s = cell_off[cl.gbase], e = cell_off[cl.gbase + 64];
+    if (s == e) return;
+
+    const double *src[6] = {F.Ex, F.Ey, F.Ez, F.Bx, F.By, F.Bz};
+    for (int t = threadIdx.x; t < GT3; t += blockDim.x) {
+        const int a = t / (GT * GT), b = (t / GT) % GT, c = t % GT;
+        const int64_t idx = ((int64_t)wrapi(cl.G0[0] - GO + a, g.N[0]) * g.N[1] + wrapi(cl.G0[1] - GO + b, g.N[1])) *
+                                g.N[2] + wrapi(cl.G0[2] - GO + c, g.N[2]);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) sF[f * GT3 + t] = (float)src[f][idx];
+    }
+    for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.f;
+    if (threadIdx.x < 64) sHist[threadIdx.x] = 0;
+    __syncthreads();
+
+    const float dtdx[3] = {(float)(g.dt * g.dinv[0]), (float)(g.dt * g.dinv[1]), (float)(g.dt * g.dinv[2])};
+    const int N = e - s;
+    const int q = INTERLEAVE ? (N + 63) >> 6 : 1;
+    const int span = INTERLEAVE ? 64 * q : N;
+    for (int kk = threadIdx.x; kk < span; kk += blockDim.x) {
+        const int rel = INTERLEAVE ? (kk & 63) * q + (kk >> 6) : kk;
+        if (rel >= N) continue;
+        const int jslot = s + rel;
+        const int i = perm[jslot];
+        const int lc = (int)((int64_t)(keysA[i] & kMask) - cl.gbase);
+        float o[3] = {A.ox[i], A.oy[i], A.oz[i]};
+        float u[3] = {A.ux[i], A.uy[i], A.uz[i]};
+        const float w = A.w[i];
+        if (lc < 0 || lc >= 64) {  // not in this cluster: corrupt state
+            atomicOr(err, 1);
+            keysB[jslot] = (13u << 27) | (uint32_t)cl.gbase;
+            B.ox[jslot] = o[0]; B.oy[jslot] = o[1]; B.oz[jslot] = o[2];
+            B.ux[jslot] = u[0]; B.uy[jslot] = u[1]; B.uz[jslot] = u[2];
+            B.w[jslot] = w;
+            atomicAdd(&sHist[0], 1);
+            continue;
+        }
+        const int l[3] = {lc >> 4, (lc >> 2) & 3, lc & 3};
+
+        // ---- shape factors and gather
+        float cp[3][5], cd[3][5], dold[3];
+        int hi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            hi[d] = o[d] >= 0.5f ? 1 : 0;
+            dold[d] = o[d] - (float)hi[d];
+            splinef<ORDER>(dold[d], cp[d]);
+            splinef<ORDER>(o[d] - 0.5f, cd[d]);
+        }
+        // tile coordinates: primal node l+hi, dual node l+1, tile origin G0-GO
+        const int p0 = l[0] + hi[0] + GO, p1 = l[1] + hi[1] + GO, p2 = l[2] + hi[2] + GO;
+        const int d0 = l[0] + 1 + GO, d1 = l[1] + 1 + GO, d2 = l[2] + 1 + GO;
+        float E[3], Bf[3];
+        E[0] = gatherf<H, GT>(cd[0], cp[1], cp[2], sF + 0 * GT3, d0, p1, p2);
+        E[1] = gatherf<H, GT>(cp[0], cd[1], cp[2], sF + 1 * GT3, p0, d1, p2);
+        E[2] = gatherf<H, GT>(cp[0], cp[1], cd[2], sF + 2 * GT3, p0, p1, d2);
+        Bf[0] = gatherf<H, GT>(cp[0], cd[1], cd[2], sF + 3 * GT3, p0, d1, d2);
+        Bf[1] = gatherf<H, GT>(cd[0], cp[1], cd[2], sF + 4 * GT3, d0, p1, d2);
+        Bf[2] = gatherf<H, GT>(cd[0], cd[1], cp[2], sF + 5 * GT3, d0, d1, p2);
+
+        // ---- Boris push, FP32.  gamma from 1 + |u|^2 with the squares summed
+        // before adding 1 and one Newton step on rsqrt.
+        const float qmh = sp.qmh;
+        float um[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) um[d] = fmaf(qmh, E[d], u[d]);
+        float g2 = 1.f + fmaf(um[0], um[0], fmaf(um[1], um[1], um[2] * um[2]));
+        float ig = rsqrtf(g2);
+        ig = ig * fmaf(-0.5f * g2 * ig, ig, 1.5f);
+        float t[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) t[d] = qmh * ig * Bf[d];
+        const float t2 = fmaf(t[0], t[0], fmaf(t[1], t[1], t[2] * t[2]));
+        const float sfac = 2.f / (1.f + t2);
+        const float up[3] = {fmaf(um[1], t[2], fmaf(-um[2], t[1], um[0])), fmaf(um[2], t[0], fmaf(-um[0], t[2], um[1])),
+                             fmaf(um[0], t[1], fmaf(-um[1], t[0], um[2]))};
+        const float sv[3] = {sfac * t[0], sfac * t[1], sfac * t[2]};
+        float un[3] = {fmaf(up[1], sv[2], fmaf(-up[2], sv[1], um[0])), fmaf(up[2], sv[0], fmaf(-up[0], sv[2], um[1])),
+                       fmaf(up[0], sv[1], fmaf(-up[1], sv[0], um[2]))};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) un[d] = fmaf(qmh, E[d], un[d]);
+        g2 = 1.f + fmaf(un[0], un[0], fmaf(un[1], un[1], un[2] * un[2]));
+        ig = rsqrtf(g2);
+        ig = ig * fmaf(-0.5f * g2 * ig, ig, 1.5f);
+
+        // ---- move: offsets in cell units, exact integer cell change
+        float on[3];
+        int cn[3], mdir[3], cw_[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float v = fmaf(un[d] * ig, dtdx[d], o[d]);
+            int step = 0;
+            if (v < 0.f) {
+                v += 1.f;
+                step = -1;
+            } else if (v >= 1.f) {
+                v -= 1.f;
+                step = 1;
+            }
+            if (v >= 1.f) v = 0.99999994f;  // v+1 rounded up from a tiny negative
+            on[d] = v;
+            cn[d] = cl.G0[d] + l[d] + step;  // unwrapped global cell
+            const int lo = cl.pc[d] * g.n[d];
+            mdir[d] = cn[d] < lo ? -1 : (cn[d] >= lo + g.n[d] ? 1 : 0);
+            cw_[d] = cn[d] < 0 ? cn[d] + g.N[d] : (cn[d] >= g.N[d] ? cn[d] - g.N[d] : cn[d]);
+        }
+
+        // ---- Esirkepov deposit into the FP32 LDS tile
+        {
+            float S0[3][ORDER + 2], DS[3][ORDER + 2];
+            int w0[3], o3[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int hn = on[d] >= 0.5f ? 1 : 0;
+                const int sh = (cn[d] - cl.G0[d] - l[d]) + hn - hi[d];  // new node - old node
+                windowf<ORDER>(dold[d], on[d] - (float)hn, sh, S0[d], DS[d], w0[d]);
+                o3[d] = l[d] + hi[d];  // old node relative to G0 == tile coordinate of stencil index 0 (JO = H+1)
+            }
+            const float cwp = sp.cw_scale * w;
+            const float cr[3] = {cwp * sp.dxdt[0], cwp * sp.dxdt[1], cwp * sp.dxdt[2]};
+            float *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
+            esirkepov3f<ORDER>(S0, DS, w0, cr, [&](int c, int a, int b, int k, float v) {
+                atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
+            });
+        }
+
+        // ---- destination key, direction of motion, store
+        int dpc[3];
+        const uint32_t gkey = global_key3(g, cw_, dpc);
+        const uint32_t dir = (uint32_t)((mdir[0] + 1) * 9 + (mdir[1] + 1) * 3 + (mdir[2] + 1));
+        keysB[jslot] = (dir << 27) | gkey;
+        B.ox[jslot] = on[0];
+        B.oy[jslot] = on[1];
+        B.oz[jslot] = on[2];
+        B.ux[jslot] = un[0];
+        B.uy[jslot] = un[1];
+        B.uz[jslot] = un[2];
+        B.w[jslot] = w;
+        const int64_t rc = (int64_t)gkey - cl.gbase;
+        if (rc >= 0 && rc < 64) atomicAdd(&sHist[rc], 1);
+        else atomicAdd(&counts[gkey], 1);
+    }
+
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int c = sHist[threadIdx.x];
+        if (c) atomicAdd(&counts[cl.gbase + threadIdx.x], c);
+    }
+    double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
+    for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
+        const float v = sJ[t];
+        if (v == 0.f) continue;
+        const int c = t / JT3, r = t % JT3;
+        const int a = r / (JT * JT), b = (r / JT) % JT, k = r % JT;
+        const int gi = wrapi(cl.G0[0] - JO + a, g.N[0]);
+        const int gj = wrapi(cl.G0[1] - JO + b, g.N[1]);
+        const int gk = wrapi(cl.G0[2] - JO + k, g.N[2]);
+        unsafeAtomicAdd(&dstJ[c][((int64_t)gi * g.N[1] + gj) * g.N[2] + gk], (double)v);
+    }
+}
+
+// FP64 global positions -> (cell key, FP32 offsets); FP32 momenta/weight.
+__global__ void k_to_mixed(Geom g, PartPtrs X, int64_t n, MixPtrs M, uint32_t *__restrict__ keys)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x[3] = {X.x[i], X.y[i], X.z[i]};
+    int c[3];
+    float off[3];
+    for (int d = 0; d < 3; ++d) {
+        const double xn = x[d] * g.dinv[d];
+        int ci = (int)floor(xn);
+        ci = ci < 0 ? 0 : (ci > g.N[d] - 1 ? g.N[d] - 1 : ci);
+        float f = (float)(xn - (double)ci);
+        f = f < 0.f ? 0.f : (f >= 1.f ? 0.99999994f : f);
+        c[d] = ci;
+        off[d] = f;
+    }
+    int pc[3];
+    keys[i] = (13u << 27) | global_key3(g, c, pc);
+    M.ox[i] = off[0];
+    M.oy[i] = off[1];
+    M.oz[i] = off[2];
+    M.ux[i] = (float)X.px[i];
+    M.uy[i] = (float)X.py[i];
+    M.uz[i] = (float)X.pz[i];
+    M.w[i] = (float)X.w[i];
+}
+
+// Sorted view -> FP64 global positions (x = (cell + offset) * dx).
+__global__ void k_from_mixed(Geom g, MixPtrs M, const uint32_t *__restrict__ keys, const int32_t *__restrict__ perm,
+                             int64_t n, PartPtrs X)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int i = perm[j];
+    const uint32_t gk = keys[i] & kMask;
+    const int p = (int)(gk / (uint32_t)g.cpp), key = (int)(gk % (uint32_t)g.cpp);
+    const int pc[3] = {p / (g.npatch[1] * g.npatch[2]), (p / g.npatch[2]) % g.npatch[1], p % g.npatch[2]};
+    const int cl = key >> 6, lc = key & 63;
+    const int cc[3] = {cl / (g.ncl[1] * g.ncl[2]), (cl / g.ncl[2]) % g.ncl[1], cl % g.ncl[2]};
+    const int l[3] = {lc >> 4, (lc >> 2) & 3, lc & 3};
+    const float off[3] = {M.ox[i], M.oy[i], M.oz[i]};
+    double xo[3];
+    for (int d = 0; d < 3; ++d) xo[d] = ((double)(pc[d] * g.n[d] + 4 * cc[d] + l[d]) + (double)off[d]) * g.dx[d];
+    X.x[j] = xo[0];
+    X.y[j] = xo[1];
+    X.z[j] = xo[2];
+    X.px[j] = M.ux[i];
+    X.py[j] = M.uy[i];
+    X.pz[j] = M.uz[i];
+    X.w[j] = M.w[i];
+}
+
+// Kinetic energy sum w (gamma - 1) with gamma - 1 = |u|^2 / (1 + gamma)
+// (no cancellation for cold particles), accumulated in FP64.
+__global__ void k_ukin_mixed(MixPtrs M, const int32_t *__restrict__ perm, int64_t n, double *partials)
+{
+    __shared__ double part[16];
+    double s = 0.;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const int i = perm[j];
+        const double ux = M.ux[i], uy = M.uy[i], uz = M.uz[i];
+        const double u2 = ux * ux + uy * uy + uz * uz;
+        s += (double)M.w[i] * (u2 / (1.0 + sqrt(1.0 + u2)));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) part[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// Charge density of the mixed particles (FP64 accumulation).
+template <int ORDER>
+__global__ void k_rho_mixed(Geom g, SpeciesKM sp, MixPtrs M, const uint32_t *__restrict__ keys,
+                            const int32_t *__restrict__ perm, int64_t n, double *__restrict__ rho)
+{
+    constexpr int H = ORDER / 2, W = 2 * H + 1;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int i = perm[j];
+    const uint32_t gk = keys[i] & kMask;
+    const int p = (int)(gk / (uint32_t)g.cpp), key = (int)(gk % (uint32_t)g.cpp);
+    const int pc[3] = {p / (g.npatch[1] * g.npatch[2]), (p / g.npatch[2]) % g.npatch[1], p % g.npatch[2]};
+    const int cl = key >> 6, lc = key & 63;
+    const int cc[3] = {cl / (g.ncl[1] * g.ncl[2]), (cl / g.ncl[2]) % g.ncl[1], cl % g.ncl[2]};
+    const int l[3] = {lc >> 4, (lc >> 2) & 3, lc & 3};
+    const float off[3] = {M.ox[i], M.oy[i], M.oz[i]};
+    double S[3][5];
+    int base[3];
+    for (int d = 0; d < 3; ++d) {
+        const int hi = off[d] >= 0.5f ? 1 : 0;
+        float w[5];
+        splinef<ORDER>(off[d] - (float)hi, w);
+        for (int a = 0; a < W; ++a) S[d][a] = w[a];
+        base[d] = pc[d] * g.n[d] + 4 * cc[d] + l[d] + hi - H;
+    }
+    const double cwv = (double)sp.cw_scale * (double)M.w[i];
+    for (int a = 0; a < W; ++a)
+        for (int b = 0; b < W; ++b)
+            for (int c = 0; c < W; ++c) {
+                const int64_t idx = ((int64_t)wrapi(base[0] + a, g.N[0]) * g.N[1] + wrapi(base[1] + b, g.N[1])) *
+                                        g.N[2] + wrapi(base[2] + c, g.N[2]);
+                unsafeAtomicAdd(&rho[idx], cwv * S[0][a] * S[1][b] * S[2][c]);
+            }
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_particles_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &A, const uint32_t *keysA,
+                            const int32_t *perm, const MixPtrs &B, uint32_t *keysB, const int32_t *cell_off,
+                            const GatherFields &F, const JPtrs &J, int *err, int32_t *counts, bool interleave,
+                            hipStream_t st)
+{
+    if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
+    const dim3 grid(g.npatches * g.clpp), block(256);
+#define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), grid, block, 0, st, g, sp, A, keysA, perm, B, keysB, \
+                                        cell_off, F, J, err, counts)
+    if (g.order == 2) {
+        if (interleave) SMI_PM(2, true);
+        else SMI_PM(2, false);
+    } else {
+        if (interleave) SMI_PM(4, true);
+        else SMI_PM(4, false);
+    }
+#undef SMI_PM
+    SMI_LAUNCH_CHECK();
+}
+
+void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_mixed, dim3(nblk(n, 256)), dim3(256), 0, st, g, X, n, M, keys);
+    SMI_LAUNCH_CHECK();
+}
+
+void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm, int64_t n,
+                       const PartPtrs &X, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_from_mixed, dim3(nblk(n, 256)), dim3(256), 0, st, g, M, keys, perm, n, X);
+    SMI_LAUNCH_CHECK();
+}
+
+void launch_ukin_mixed(const MixPtrs &M, const int32_t *perm, int64_t n, double *partials, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ukin_mixed, dim3(nblocks), dim3(256), 0, st, M, perm, n, partials);
+    SMI_LAUNCH_CHECK();
+}
+
+void launch_rho_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,
+                      int64_t n, double *rho, hipStream_t st)
+{
+    if (n == 0) return;
+    if (g.order == 2)
+        hipLaunchKernelGGL(k_rho_mixed<2>, dim3(nblk(n, 256)), dim3(256), 0, st, g, sp, M, keys, perm, n, rho);
+    else
+        hipLaunchKernelGGL(k_rho_mixed<4>, dim3(nblk(n, 256)), dim3(256), 0, st, g, sp, M, keys, perm, n, rho);
+    SMI_LAUNCH_CHECK();
+}
+
+}  // namespace smi

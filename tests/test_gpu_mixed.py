@@ -1,0 +1,97 @@
+"""Mixed-precision form (FP32 particles, int32 cell + FP32 offset, FP64
+fields and FP64 J reduction) against the plain FP64 form and the oracle on
+identical seeded initial conditions.
+
+Tolerances follow the workload's targets for the error metrics: relative
+field and kinetic energy error against the FP64 run, Gauss-law drift
+|div E - rho|_max / max|rho|; the particle bookkeeping (counts, ownership,
+sorted keys) must be exact.
+"""
+import numpy as np
+import pytest
+
+import smilei_amd as sa
+from oracle import pyoracle as O
+from tests.plasma import cfl_dt, load, thermal
+
+pytestmark = pytest.mark.gpu
+
+SPECIES = [("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))]
+
+
+def make(precision, order=2, shape=(32, 32, 32), patch=(16, 16, 16), ppc=8, seed=3, interleave=1):
+    dx = (0.2, 0.2, 0.2)
+    dt = cfl_dt(dx)
+    sim = sa.Simulation("3Dcartesian", order, shape, patch, dx, dt, precision=precision,
+                        deposit=sa.DEPOSIT_ATOMIC)
+    if precision == sa.MIXED:
+        sim._L.smi_set_tuning(sim.h, 1, interleave)
+    parts = []
+    for k, (name, m, q, sig) in enumerate(SPECIES):
+        p = thermal(shape, dx, ppc, sig, seed + k)
+        load(sim, sim.add_species(name, m, q, len(p["x"])), p)
+        parts.append(p)
+    return sim, parts, dx
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_mixed_bookkeeping_exact(order):
+    shape, patch = ((32, 32, 32), (16, 16, 16)) if order == 2 else ((24, 24, 24), (12, 12, 12))
+    sim, parts, dx = make(sa.MIXED, order, shape, patch)
+    sim.step(20)
+    sim.synchronize()
+    nps = [shape[d] // patch[d] for d in range(3)]
+    for s in range(2):
+        p = sim.get_particles(s)
+        assert len(p["x"]) == len(parts[s]["x"])
+        np.testing.assert_allclose(np.sort(p["w"]), np.sort(parts[s]["w"].astype(np.float32)), rtol=0)
+        pos = [p["x"], p["y"], p["z"]]
+        for d in range(3):
+            assert np.all(pos[d] >= 0) and np.all(pos[d] < shape[d] * dx[d] * (1 + 1e-12))
+        pc = [np.minimum((pos[d] / (patch[d] * dx[d])).astype(int), nps[d] - 1) for d in range(3)]
+        owner = (pc[0] * nps[1] + pc[1]) * nps[2] + pc[2]
+        # the split representation is exact: no position sits in a neighbour's cell
+        np.testing.assert_array_equal(p["patch"], owner)
+        for q in range(sim.npatch):
+            assert np.all(np.diff(p["key"][p["patch"] == q]) >= 0)
+        np.testing.assert_array_equal(sim.patch_counts(s), np.bincount(p["patch"], minlength=sim.npatch))
+
+
+@pytest.mark.parametrize("interleave", [1, 0])
+def test_mixed_energies_and_gauss_vs_fp64(interleave):
+    nsteps = 100
+    m, _, _ = make(sa.MIXED, interleave=interleave)
+    f, _, _ = make(sa.PLAIN_FP64)
+    gm0 = m.gauss()
+    rho0 = np.abs(m.rho()).max()
+    e0 = f.energies()
+    m.step(nsteps)
+    f.step(nsteps)
+    em, ef = m.energies(), f.energies()
+    # kinetic energies of both species and the total
+    np.testing.assert_allclose(em[1:], ef[1:], rtol=1e-5)
+    np.testing.assert_allclose(em.sum(), ef.sum(), rtol=1e-5)
+    # the field energy grows from zero (thermal noise): compare to the total
+    assert abs(em[0] - ef[0]) <= 1e-5 * ef.sum()
+    assert abs(ef.sum() / e0.sum() - 1) < 5e-2
+    drift = np.abs(m.gauss() - gm0).max() / rho0
+    assert drift < 1e-6, drift
+
+
+def test_mixed_first_step_matches_oracle_fields():
+    """One step from identical particles: the mixed J and E agree with the
+    FP64 oracle to FP32 accuracy (relative to max |J|)."""
+    shape, patch, dx = (32, 16, 16), (8, 8, 8), (0.2, 0.2, 0.2)
+    dt = cfl_dt(dx)
+    sim = sa.Simulation("3Dcartesian", 2, shape, patch, dx, dt, precision=sa.MIXED)
+    o = O.OracleSim(3, 2, shape, patch, dx, dt)
+    for k, (name, mass, q, sig) in enumerate(SPECIES):
+        p = thermal(shape, dx, 4, sig * 4, 20 + k)
+        load(sim, sim.add_species(name, mass, q, len(p["x"])), p)
+        o.set_particles(o.add_species(mass, q), p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    sim.step(1)
+    o.step(1)
+    for fld in ("Jx", "Jy", "Jz", "Ex", "Ey", "Ez"):
+        ref = o.get_field(fld)
+        err = np.abs(sim.get_field(fld) - ref).max() / np.abs(ref).max()
+        assert err < 2e-5, (fld, err)
