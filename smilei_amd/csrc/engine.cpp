@@ -168,6 +168,21 @@ void enqueue_step(smi_context *c, bool timed)
 
 void flip(smi_context *c) { c->cur ^= 1; }
 
+// Fixed-point quantum of the mixed J tile: 2^-32 of the largest current
+// contribution |q| w_max / V * max(dx/dt) of the species.
+void set_quantum(const Geom &g, SpeciesState &s, double w_max)
+{
+    double cell_volume = 1.0, dxdt = 0.;
+    for (int d = 0; d < g.nd; ++d) {
+        cell_volume *= g.dx[d];
+        dxdt = std::max(dxdt, g.dx[d] / g.dt);
+    }
+    double cmax = std::fabs(s.charge) / cell_volume * w_max * dxdt;
+    if (!(cmax > 0)) cmax = 1.0;
+    s.km.quantum = std::ldexp(cmax, -32);
+    s.km.inv_quantum = (float)(1.0 / s.km.quantum);
+}
+
 int fail(const Error &e)
 {
     g_last_error = e.what();
@@ -343,6 +358,7 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.km.qmh = (float)s.k.qmh;
     s.km.cw_scale = (float)s.k.cw_scale;
     for (int d = 0; d < 3; ++d) s.km.dxdt[d] = (float)(c->g.dx[d] / c->g.dt);
+    set_quantum(c->g, s, 1.0);
     c->sp.push_back(s);
     // deterministic-deposit descriptors for both ping-pong parities
     (void)hipFree(c->d_det);
@@ -376,6 +392,9 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
     SMI_HIP(hipStreamSynchronize(c->st));
     S.n = n;
     if (c->mixed) {
+        double wmax = 0.;
+        for (int64_t i = 0; i < n; ++i) wmax = std::max(wmax, std::fabs(w[i]));
+        set_quantum(c->g, S, wmax);
         // stage FP64 on the device, convert to cell + FP32 offset
         std::vector<double *> tmp(7, nullptr);
         for (int q = 0; q < 7; ++q) {
@@ -441,6 +460,7 @@ int smi_init_species(smi_context *c, int s, int ppc, double density, double sigm
     double cell_volume = 1.0;
     for (int d = 0; d < g.nd; ++d) cell_volume *= g.dx[d];
     const MixPtrs none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (c->mixed) set_quantum(g, S, std::fabs(density * cell_volume / ppc));
     launch_init(g, S.view(c->cur), c->mixed ? S.mview(c->cur) : none, S.keys[c->cur], n, ppc, d_cells,
                 density * cell_volume / ppc, sigma, drift_ux, seed * 0x100000001B3ull + (uint64_t)s, c->st);
     S.n = n;
@@ -581,6 +601,9 @@ int smi_synchronize(smi_context *c)
     SMI_HIP(hipStreamSynchronize(c->st));
     int e = 0;
     SMI_HIP(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e & 2)
+        throw Error(SMI_ERR_STATE, "more than 32767 particles in one 4^3-cell cluster: beyond the range of the "
+                                   "mixed form's fixed-point current tile");
     if (e)
         throw Error(SMI_ERR_STATE, "a particle left its cluster stencil (moved more than one cell per step, "
                                    "non-finite position or corrupt input); the step skipped it");

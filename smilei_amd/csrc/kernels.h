@@ -20,9 +20,11 @@ struct MixPtrs {
 
 // Species constants of the mixed kernel (FP32).
 struct SpeciesKM {
-    float qmh;       // charge/mass * dt/2
-    float cw_scale;  // charge / V_cell
-    float dxdt[3];   // dx/dt, dy/dt, dz/dt
+    float qmh;          // charge/mass * dt/2
+    float cw_scale;     // charge / V_cell
+    float dxdt[3];      // dx/dt, dy/dt, dz/dt
+    float inv_quantum;  // 1 / quantum of the fixed-point J tile
+    double quantum;     // 2^-32 * the largest current contribution of the species
 };
 
 // Fields read by the gather (E and the centred B_m).

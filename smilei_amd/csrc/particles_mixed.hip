@@ -91,22 +91,47 @@ __device__ inline void windowf(float dold, float dnew, int sh, float *S0w, float
     }
 }
 
+// Fixed-point accumulation of the J tile.  A contribution c, already
+// scaled to units of the quantum 2^-32 * cr_max of its species, is rounded
+// to an integer k by one FP64 add of 1.5*2^52 (the sum's mantissa then holds
+// k); the raw bits of that double, 0x4338000000000000 + k, are summed with
+// an integer LDS atomic.  The constant only reaches bits 48..63, so the low
+// 48 bits of a tile word hold sum(k) exactly while |sum(k)| < 2^47, i.e.
+// for fewer than 2^15 contributions per point (|k| <= 2^32 because
+// |c| <= cr).  Integer adds make the tile sums independent of the order of
+// the atomics; the quantum is 2.3e-10 of the largest contribution.
+constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
+constexpr int kMaxClusterParticles = 32767;
+
+__device__ inline double fixed_decode(unsigned long long v, double quantum)
+{
+    const long long k = ((long long)(v << 16)) >> 16;  // sign-extend the low 48 bits
+    return (double)k * quantum;
+}
+
 // Windowed 3D Esirkepov with W(b,c) = S0b (S0c + DSc/2) + DSb (S0c/2 + DSc/3)
-// (the reference's four-term weight, factorised), FP32.
-template <int ORDER, class Add>
-__device__ inline void esirkepov3f(const float (&S0)[3][ORDER + 2], const float (&DS)[3][ORDER + 2], const int *w0,
-                                   const float *cr, Add add)
+// (the reference's four-term weight, factorised), FP32 weights, FP64 prefix
+// sums along the current's axis, fixed-point tile.  Every window row is
+// written, including the one past a window that starts at stencil index 1
+// (its value is the rounding residue of sum(DS), ~1e-7 of the column), so
+// the loop has no lane-dependent branch; the tile is one cell larger to hold
+// it.  tile(c, i, j, k) gives the LDS word of component c at window
+// indices relative to the particle's stencil origin.
+template <int ORDER, class Tile>
+__device__ inline void esirkepov3_fixed(const float (&S0)[3][ORDER + 2], const float (&DS)[3][ORDER + 2],
+                                        const int *w0, const float *crq, Tile tile)
 {
     constexpr int W = ORDER + 2;
     float P[3][W], Q[3][W];
+    double DSd[3][W];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int m = 0; m < W; ++m) {
             P[d][m] = fmaf(0.5f, DS[d][m], S0[d][m]);
             Q[d][m] = fmaf(1.f / 3.f, DS[d][m], 0.5f * S0[d][m]);
+            DSd[d][m] = (double)DS[d][m];
         }
-    // component c along axis c, transverse axes (b, e) cyclic
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int bx = (c + 1) % 3, ex = (c + 2) % 3;
@@ -114,19 +139,17 @@ __device__ inline void esirkepov3f(const float (&S0)[3][ORDER + 2], const float 
         for (int a = 0; a < W; ++a)
 #pragma unroll
             for (int b = 0; b < W; ++b) {
-                const float Wt = fmaf(S0[bx][a], P[ex][b], DS[bx][a] * Q[ex][b]);
-                const float t = cr[c] * Wt;
-                float acc = 0.f;
+                const double t = (double)(crq[c] * fmaf(S0[bx][a], P[ex][b], DS[bx][a] * Q[ex][b]));
+                double acc = 0.;
 #pragma unroll
                 for (int r = 1; r <= W; ++r) {
-                    acc = fmaf(-DS[c][r - 1], t, acc);
-                    if (r < W || w0[c] == 0) {
-                        int idx[3];
-                        idx[c] = w0[c] + r;
-                        idx[bx] = w0[bx] + a;
-                        idx[ex] = w0[ex] + b;
-                        add(c, idx[0], idx[1], idx[2], acc);
-                    }
+                    acc = fma(-DSd[c][r - 1], t, acc);
+                    int idx[3];
+                    idx[c] = w0[c] + r;
+                    idx[bx] = w0[bx] + a;
+                    idx[ex] = w0[ex] + b;
+                    atomicAdd(tile(c, idx[0], idx[1], idx[2]),
+                              (unsigned long long)__double_as_longlong(acc + kRound));
                 }
             }
     }
@@ -171,10 +194,11 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
                   JPtrs J, int *__restrict__ err, int32_t *__restrict__ counts)
 {
     constexpr int H = ORDER / 2;
-    constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
+    // J tile: stencil rows [c0-H-1, c0+4+H+1] plus the residue row
+    constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 4;
     constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
     __shared__ float sF[6 * GT3];
-    __shared__ float sJ[3 * JT3];
+    __shared__ unsigned long long sJ[3 * JT3];
     __shared__ int sHist[64];
 
     const Cl cl = cluster_of(g, blockIdx.x);
@@ -189,8 +213,9 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
 #pragma unroll
         for (int f = 0; f < 6; ++f) sF[f * GT3 + t] = (float)src[f][idx];
     }
-    for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.f;
+    for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0ull;
     if (threadIdx.x < 64) sHist[threadIdx.x] = 0;
+    if (threadIdx.x == 0 && e - s > kMaxClusterParticles) atomicOr(err, 2);  // fixed-point headroom
     __syncthreads();
 
     const float dtdx[3] = {(float)(g.dt * g.dinv[0]), (float)(g.dt * g.dinv[1]), (float)(g.dt * g.dinv[2])};
@@ -296,11 +321,12 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
                 windowf<ORDER>(dold[d], on[d] - (float)hn, sh, S0[d], DS[d], w0[d]);
                 o3[d] = l[d] + hi[d];  // old node relative to G0 == tile coordinate of stencil index 0 (JO = H+1)
             }
-            const float cwp = sp.cw_scale * w;
-            const float cr[3] = {cwp * sp.dxdt[0], cwp * sp.dxdt[1], cwp * sp.dxdt[2]};
-            float *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
-            esirkepov3f<ORDER>(S0, DS, w0, cr, [&](int c, int a, int b, int k, float v) {
-                atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
+            // contributions in units of the species' fixed-point quantum
+            const float cwq = sp.cw_scale * w * sp.inv_quantum;
+            const float crq[3] = {cwq * sp.dxdt[0], cwq * sp.dxdt[1], cwq * sp.dxdt[2]};
+            unsigned long long *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
+            esirkepov3_fixed<ORDER>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
+                return &sJb[c * JT3 + (a * JT + b) * JT + k];
             });
         }
 
@@ -327,15 +353,16 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
         if (c) atomicAdd(&counts[cl.gbase + threadIdx.x], c);
     }
     double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
+    const double quantum = sp.quantum;
     for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
-        const float v = sJ[t];
-        if (v == 0.f) continue;
+        const double v = fixed_decode(sJ[t], quantum);
+        if (v == 0.) continue;
         const int c = t / JT3, r = t % JT3;
         const int a = r / (JT * JT), b = (r / JT) % JT, k = r % JT;
         const int gi = wrapi(cl.G0[0] - JO + a, g.N[0]);
         const int gj = wrapi(cl.G0[1] - JO + b, g.N[1]);
         const int gk = wrapi(cl.G0[2] - JO + k, g.N[2]);
-        unsafeAtomicAdd(&dstJ[c][((int64_t)gi * g.N[1] + gj) * g.N[2] + gk], (double)v);
+        unsafeAtomicAdd(&dstJ[c][((int64_t)gi * g.N[1] + gj) * g.N[2] + gk], v);
     }
 }
 
