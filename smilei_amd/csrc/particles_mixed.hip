@@ -9,9 +9,9 @@
 // Arithmetic: shape factors, the gather, the Boris rotation and the
 // Esirkepov contributions in FP32 with fused multiply-adds; the gathered
 // E/B are staged from the FP64 field arrays; each cluster's J tile is
-// accumulated in FP32 in LDS and flushed with FP64 atomics into the FP64 J
-// arrays (the global J reduction stays FP64, as do E, B and the Yee
-// update).  The cell change and the periodic wrap are exact integer
+// accumulated in LDS as 64-bit fixed point (exact integer sums) and flushed
+// with FP64 atomics into the FP64 J arrays (the global J reduction stays
+// FP64, as do E, B and the Yee update).  The cell change and the periodic wrap are exact integer
 // operations, so no rounding can misplace a particle at a patch face.
 //
 // Same call chain as the FP64 kernel: Interpolator3D{2,4}Order (gather),
@@ -111,38 +111,42 @@ __device__ inline double fixed_decode(unsigned long long v, double quantum)
 
 // Windowed 3D Esirkepov with W(b,c) = S0b (S0c + DSc/2) + DSb (S0c/2 + DSc/3)
 // (the reference's four-term weight, factorised), FP32 weights, FP64 prefix
-// sums along the current's axis, fixed-point tile.  Every window row is
-// written, including the one past a window that starts at stencil index 1
-// (its value is the rounding residue of sum(DS), ~1e-7 of the column), so
-// the loop has no lane-dependent branch; the tile is one cell larger to hold
-// it.  tile(c, i, j, k) gives the LDS word of component c at window
-// indices relative to the particle's stencil origin.
-template <int ORDER, class Tile>
+// sums along the current's axis, fixed-point tile.  Only the non-zero part
+// of the stencil is visited: NT window entries across the current (entries
+// past the union of the old and new supports are exact zeros) and NA faces
+// along it.  The face after the last window entry is left out: its prefix
+// sum is sum(DS) over the whole window, zero up to the FP32 rounding of the
+// shape factors.  A particle whose nearest node is unchanged on every axis
+// has NT = ORDER+1, NA = ORDER (the two supports coincide); otherwise
+// NT = ORDER+2, NA = ORDER+1.  tile(c, i, j, k) gives the LDS word of
+// component c at window indices relative to the particle's stencil origin.
+template <int ORDER, int NT, int NA, class Tile>
 __device__ inline void esirkepov3_fixed(const float (&S0)[3][ORDER + 2], const float (&DS)[3][ORDER + 2],
                                         const int *w0, const float *crq, Tile tile)
 {
-    constexpr int W = ORDER + 2;
-    float P[3][W], Q[3][W];
-    double DSd[3][W];
+    float P[3][NT], Q[3][NT];
+    double DSd[3][NA];
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 3; ++d) {
 #pragma unroll
-        for (int m = 0; m < W; ++m) {
+        for (int m = 0; m < NT; ++m) {
             P[d][m] = fmaf(0.5f, DS[d][m], S0[d][m]);
             Q[d][m] = fmaf(1.f / 3.f, DS[d][m], 0.5f * S0[d][m]);
-            DSd[d][m] = (double)DS[d][m];
         }
+#pragma unroll
+        for (int m = 0; m < NA; ++m) DSd[d][m] = (double)DS[d][m];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int bx = (c + 1) % 3, ex = (c + 2) % 3;
 #pragma unroll
-        for (int a = 0; a < W; ++a)
+        for (int a = 0; a < NT; ++a)
 #pragma unroll
-            for (int b = 0; b < W; ++b) {
+            for (int b = 0; b < NT; ++b) {
                 const double t = (double)(crq[c] * fmaf(S0[bx][a], P[ex][b], DS[bx][a] * Q[ex][b]));
                 double acc = 0.;
 #pragma unroll
-                for (int r = 1; r <= W; ++r) {
+                for (int r = 1; r <= NA; ++r) {
                     acc = fma(-DSd[c][r - 1], t, acc);
                     int idx[3];
                     idx[c] = w0[c] + r;
@@ -188,14 +192,14 @@ __device__ inline uint32_t global_key3(const Geom &g, const int *c, int *pc)
 }
 
 template <int ORDER, bool INTERLEAVE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, ORDER == 2 ? 4 : 2)
 k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ keysA, const int32_t *__restrict__ perm,
                   MixPtrs B, uint32_t *__restrict__ keysB, const int32_t *__restrict__ cell_off, GatherFields F,
                   JPtrs J, int *__restrict__ err, int32_t *__restrict__ counts)
 {
     constexpr int H = ORDER / 2;
-    // J tile: stencil rows [c0-H-1, c0+4+H+1] plus the residue row
-    constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 4;
+    // J tile: stencil rows [c0-H-1, c0+4+H]
+    constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
     constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
     __shared__ float sF[6 * GT3];
     __shared__ unsigned long long sJ[3 * JT3];
@@ -313,11 +317,12 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
         // ---- Esirkepov deposit into the FP32 LDS tile
         {
             float S0[3][ORDER + 2], DS[3][ORDER + 2];
-            int w0[3], o3[3];
+            int w0[3], o3[3], moved = 0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int hn = on[d] >= 0.5f ? 1 : 0;
                 const int sh = (cn[d] - cl.G0[d] - l[d]) + hn - hi[d];  // new node - old node
+                moved |= sh;
                 windowf<ORDER>(dold[d], on[d] - (float)hn, sh, S0[d], DS[d], w0[d]);
                 o3[d] = l[d] + hi[d];  // old node relative to G0 == tile coordinate of stencil index 0 (JO = H+1)
             }
@@ -325,9 +330,9 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
             const float cwq = sp.cw_scale * w * sp.inv_quantum;
             const float crq[3] = {cwq * sp.dxdt[0], cwq * sp.dxdt[1], cwq * sp.dxdt[2]};
             unsigned long long *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
-            esirkepov3_fixed<ORDER>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
-                return &sJb[c * JT3 + (a * JT + b) * JT + k];
-            });
+            const auto tile = [&](int c, int a, int b, int k) { return &sJb[c * JT3 + (a * JT + b) * JT + k]; };
+            if (moved == 0) esirkepov3_fixed<ORDER, ORDER + 1, ORDER>(S0, DS, w0, crq, tile);
+            else esirkepov3_fixed<ORDER, ORDER + 2, ORDER + 1>(S0, DS, w0, crq, tile);
         }
 
         // ---- destination key, direction of motion, store
