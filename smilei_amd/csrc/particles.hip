@@ -204,19 +204,21 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
             const double cw = sp.cw_scale * w;
             if constexpr (ND == 3) {
                 double S0[3][ORDER + 2], DS[3][ORDER + 2];
-                int w0[3];
+                int w0[3], sh[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
-                    esirkepov_window<ORDER>(dold[d], xn[d] * g.dinv[d], ip[d], S0[d], DS[d], w0[d]);
+                    esirkepov_window<ORDER>(dold[d], xn[d] * g.dinv[d], ip[d], S0[d], DS[d], w0[d], &sh[d]);
                 const double cr[3] = {cw * (g.dx[0] / g.dt), cw * (g.dx[1] / g.dt), cw * (g.dx[2] / g.dt)};
                 // tile coords of stencil index 0 along each axis
                 const int o0 = ip[0] - (H + 1) - (cl.G0[0] - TL::JO);
                 const int o1 = ip[1] - (H + 1) - (cl.G0[1] - TL::JO);
                 const int o2 = ip[2] - (H + 1) - (cl.G0[2] - TL::JO);
                 double *sJb = sJ + (o0 * JT + o1) * JT + o2;
-                esirkepov3d_win<ORDER>(S0, DS, w0, cr, [&](int c, int a, int b, int k, double v) {
+                const auto tile_add = [&](int c, int a, int b, int k, double v) {
                     atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
-                });
+                };
+                if ((sh[0] | sh[1] | sh[2]) == 0) esirkepov3d_win<ORDER, ORDER + 1>(S0, DS, w0, cr, tile_add);
+                else esirkepov3d_win<ORDER>(S0, DS, w0, cr, tile_add);
             } else {
                 double S0[3][7], S1[3][7], DS[3][7];
 #pragma unroll

@@ -263,7 +263,8 @@ __device__ inline void esirkepov3d(const double (&S0)[3][7], const double (&DS)[
 // bit-identical to the reference's.  The window arrays are filled with
 // compile-time indices only (selects, no dynamic register indexing).
 template <int ORDER>
-__device__ inline void esirkepov_window(double dold, double pn_new, int ipo, double *S0w, double *DSw, int &w0)
+__device__ inline void esirkepov_window(double dold, double pn_new, int ipo, double *S0w, double *DSw, int &w0,
+                                        int *shift = nullptr)
 {
     constexpr int W = ORDER + 2;
     double s[ORDER + 1], nw[ORDER + 1];
@@ -272,6 +273,7 @@ __device__ inline void esirkepov_window(double dold, double pn_new, int ipo, dou
     spline<ORDER>(pn_new - (double)ip, nw);
     const int sh = ip - ipo;
     w0 = sh < 0 ? 0 : 1;
+    if (shift) *shift = sh;
 #pragma unroll
     for (int m = 0; m < W; ++m) {
         const double s0_w0 = m == 0 ? 0. : s[m - 1];     // window starting at stencil 0
@@ -288,17 +290,20 @@ __device__ inline void esirkepov_window(double dold, double pn_new, int ipo, dou
 // Windowed walk of the 3D Esirkepov contributions (same values as
 // esirkepov3d for every point it visits; the points it skips receive exact
 // zeros there).  add(component, i, j, k, value) gets reference stencil
-// indices (0..ORDER+2) along the three axes.
-template <int ORDER, class Add>
+// indices (0..ORDER+2) along the three axes.  NT < W narrows the columns
+// across the current to the first NT window entries; a particle whose
+// nearest node is unchanged on every axis has S0 = DS = 0 past entry ORDER,
+// so NT = ORDER + 1 skips exact zeros only.
+template <int ORDER, int NT = ORDER + 2, class Add>
 __device__ inline void esirkepov3d_win(const double (&S0)[3][ORDER + 2], const double (&DS)[3][ORDER + 2],
                                        const int *w0, const double *cr, Add add)
 {
     constexpr int W = ORDER + 2, S = ORDER + 3;
     // Jx: columns (y, z) of the window, prefix along x
 #pragma unroll
-    for (int a = 0; a < W; ++a)
+    for (int a = 0; a < NT; ++a)
 #pragma unroll
-        for (int b = 0; b < W; ++b) {
+        for (int b = 0; b < NT; ++b) {
             const double Wt = weight3(S0[1], DS[1], S0[2], DS[2], a, b);
             const double t = cr[0] * Wt;
             double acc = 0.;
@@ -311,9 +316,9 @@ __device__ inline void esirkepov3d_win(const double (&S0)[3][ORDER + 2], const d
         }
     // Jy: columns (x, z), prefix along y
 #pragma unroll
-    for (int a = 0; a < W; ++a)
+    for (int a = 0; a < NT; ++a)
 #pragma unroll
-        for (int b = 0; b < W; ++b) {
+        for (int b = 0; b < NT; ++b) {
             const double Wt = weight3(S0[2], DS[2], S0[0], DS[0], b, a);
             const double t = cr[1] * Wt;
             double acc = 0.;
@@ -326,9 +331,9 @@ __device__ inline void esirkepov3d_win(const double (&S0)[3][ORDER + 2], const d
         }
     // Jz: columns (x, y), prefix along z
 #pragma unroll
-    for (int a = 0; a < W; ++a)
+    for (int a = 0; a < NT; ++a)
 #pragma unroll
-        for (int b = 0; b < W; ++b) {
+        for (int b = 0; b < NT; ++b) {
             const double Wt = weight3(S0[0], DS[0], S0[1], DS[1], a, b);
             const double t = cr[2] * Wt;
             double acc = 0.;
