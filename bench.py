@@ -1,22 +1,30 @@
 #!/usr/bin/env python3
 """Particle-pushes/s of the PIC time step on one MI355X.
 
-Workload (the plain-FP64 3D thermal plasma): 256x128x128 cells, dx = 0.2,
-dt = 0.95*CFL, 16^3-cell patches (1024 patches), electrons and protons
-(m = 1836) at 8 particles/cell each = 67,108,864 particles, uniform random
-positions, Gaussian momenta sigma = 0.05 m_e c (protons: same temperature,
-sigma = 0.05/sqrt(1836) in m_p c), 2nd-order shapes, Yee, periodic.  The
-particles are initialised in HBM; one "step" is the full PIC step (gather,
-Boris push, Esirkepov deposit, cell sort + patch exchange, Yee update).
+Default workload (3D beam-plasma, 1.08e9 particles): 1024x256x256 cells,
+dx = 0.25, dt = 0.95*CFL, 32x16x16-cell patches (8192 patches); electron
+and proton background at 8 particles/cell each, thermal sigma = 0.02 m_e c
+(protons: same temperature, 0.02/sqrt(1836) in m_p c); a cold electron beam,
+density 0.1, gamma = 50 along x (u_x = sqrt(50^2 - 1)), 8 particles/cell in
+the cells whose centre lies within 16 cells of the axis.  2nd-order shapes,
+Yee, periodic.  `--workload thermal` selects the 3D thermal plasma
+(256x128x128 cells, dx = 0.2, 16^3 patches, e-/p+ 8 ppc, sigma 0.05 m_e c,
+67,108,864 particles).
+
+The particles are initialised in HBM; one "step" is the full PIC step
+(gather, Boris push, Esirkepov deposit, cell sort + patch exchange, Yee
+update) over every species.  Both precision forms run in turn (mixed first:
+it is the headline); each is timed over the same number of steps.
 
 Prints one JSON line.  `value` = particles * steps / wall time of the timed
 steps.  `roofline` rates the dominant kernel (the fused gather-push-deposit
-kernel) against HBM: algorithmic bytes = 108 B per particle push (read
-x,y,z,ux,uy,uz,w = 56 B; write x,y,z,ux,uy,uz = 48 B; write the cell key
-4 B) divided by that kernel's device time, measured with HIP events on the
-library's stream over the timed steps.  `cpu_baseline` times the CPU
-oracle (OpenMP over patches, the reference's CPU structure) on a bounded
-sub-box of the same plasma.
+kernel) against HBM: algorithmic bytes per push (mixed: read cell key,
+3 FP32 offsets, 3 FP32 momenta, FP32 weight = 32 B, write key, offsets and
+momenta = 28 B: 60 B; FP64: read x,y,z,ux,uy,uz,w = 56 B, write x,y,z,
+ux,uy,uz = 48 B, write the key 4 B: 108 B) divided by that kernel's device
+time, measured with HIP events on the library's stream over the timed
+steps.  `cpu_baseline` times the CPU oracle (OpenMP over patches, the
+reference's CPU structure) on a bounded sub-box of the same plasma.
 """
 import argparse
 import json
@@ -38,11 +46,32 @@ def cfl_dt(dx, factor=0.95):
     return factor / np.sqrt(sum(1.0 / d ** 2 for d in dx))
 
 
-def workload():
-    shape = (256, 128, 128)
-    dx = (0.2, 0.2, 0.2)
-    return dict(shape=shape, patch=(16, 16, 16), dx=dx, dt=cfl_dt(dx), ppc=8,
-                species=[("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))])
+def workload(name="beam", ppc=8):
+    """Species: (name, mass, charge, sigma, ppc, density, drift_ux, radius)."""
+    if name == "thermal":
+        shape = (256, 128, 128)
+        dx = (0.2, 0.2, 0.2)
+        return dict(name=name, shape=shape, patch=(16, 16, 16), dx=dx, dt=cfl_dt(dx),
+                    species=[("electron", 1.0, -1.0, 0.05, 8, 1.0, 0.0, 0.0),
+                             ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0), 8, 1.0, 0.0, 0.0)],
+                    desc="3D thermal plasma 256x128x128 cells, dx=0.2, 16^3-cell patches, e-/p+ 8 ppc each, "
+                         "sigma 0.05 m_e c")
+    shape = (1024, 256, 256)
+    dx = (0.25, 0.25, 0.25)
+    ub = float(np.sqrt(50.0 ** 2 - 1.0))
+    return dict(name=name, shape=shape, patch=(32, 16, 16), dx=dx, dt=cfl_dt(dx),
+                species=[("electron", 1.0, -1.0, 0.02, ppc, 1.0, 0.0, 0.0),
+                         ("proton", 1836.0, 1.0, 0.02 / np.sqrt(1836.0), ppc, 1.0, 0.0, 0.0),
+                         ("beam", 1.0, -1.0, 0.0, 8, 0.1, ub, 16.0)],
+                desc="3D beam-plasma 1024x256x256 cells, dx=0.25, 32x16x16-cell patches, e-/p+ background "
+                     "%d ppc each sigma 0.02 m_e c, cold e- beam gamma=50 n=0.1 8 ppc r<=16 cells" % ppc)
+
+
+def cylinder_cells(shape, radius):
+    """Number of cells whose centre lies within `radius` cells of the x axis."""
+    j = np.arange(shape[1]) + 0.5 - 0.5 * shape[1]
+    k = np.arange(shape[2]) + 0.5 - 0.5 * shape[2]
+    return int(np.count_nonzero(j[:, None] ** 2 + k[None, :] ** 2 <= radius * radius)) * shape[0]
 
 
 def run_gpu(args, W, form):
@@ -51,10 +80,14 @@ def run_gpu(args, W, form):
     sim = sa.Simulation("3Dcartesian", 2, W["shape"], W["patch"], W["dx"], W["dt"],
                         precision=prec, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
     ncell = int(np.prod(W["shape"]))
-    for k, (name, m, q, sig) in enumerate(W["species"]):
-        sid = sim.add_species(name, m, q, ncell * W["ppc"])
-        sim.init_species(sid, W["ppc"], 1.0, sig, seed=1 + k)
-    npart = sum(sim.num_particles(s) for s in range(len(W["species"])))
+    per_species = []
+    for k, (name, m, q, sig, ppc, dens, drift, radius) in enumerate(W["species"]):
+        cells = cylinder_cells(W["shape"], radius) if radius > 0 else ncell
+        sid = sim.add_species(name, m, q, cells * ppc)
+        sim.init_species(sid, ppc, dens, sig, drift_ux=drift, radius=radius, seed=1 + k)
+        per_species.append(sim.num_particles(sid))
+    npart = sum(per_species)
+    e0 = sim.energies()
     sim.step(args.warmup)
     sim.synchronize()
     # timed region: eager steps with HIP events around each kernel class
@@ -73,21 +106,23 @@ def run_gpu(args, W, form):
     tg1 = time.perf_counter()
     e = sim.energies()
     sim.close()
-    return dict(npart=npart, wall=t1 - t0, wall_graph=tg1 - tg0, kernel_ms=acc, energies=e.tolist())
+    return dict(npart=npart, per_species=per_species, wall=t1 - t0, wall_graph=tg1 - tg0, kernel_ms=acc,
+                energies=e.tolist(), energies0=e0.tolist())
 
 
 def run_cpu_baseline(args, W):
-    """The oracle on a 64^3-cell sub-box of the same plasma (same dx, dt,
-    patch size and ppc): 4.19M particles, a few steps, all host cores."""
+    """The oracle on a sub-box of the same plasma (same dx, dt, patch size,
+    ppc and species; for the beam-plasma a 128x64x64-cell box with the beam
+    on its axis), a few steps, all host cores."""
     from oracle import pyoracle as O
     from tests.plasma import thermal
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     O.set_threads(threads)
-    shape = (64, 64, 64)
+    shape = (64, 64, 64) if W["name"] == "thermal" else (128, 64, 64)
     S = O.OracleSim(3, 2, shape, W["patch"], W["dx"], W["dt"])
     n = 0
-    for k, (name, m, q, sig) in enumerate(W["species"]):
-        p = thermal(shape, W["dx"], W["ppc"], sig, 100 + k)
+    for k, (name, m, q, sig, ppc, dens, drift, radius) in enumerate(W["species"]):
+        p = thermal(shape, W["dx"], ppc, sig, 100 + k, density=dens, drift_ux=drift, radius=radius)
         sid = S.add_species(m, q)
         S.set_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
         n += len(p["x"])
@@ -102,9 +137,9 @@ def run_cpu_baseline(args, W):
     dt = time.perf_counter() - t0
     rate = n * steps / dt
     return dict(value=rate, unit="particle-pushes/s", cores=threads, kind="port",
-                sample="CPU oracle, %dx%dx%d-cell sub-box of the same plasma (16^3 patches, 8+8 ppc, %d particles), "
-                       "%d steps, OpenMP over patches; %.1f ns*core/push" % (shape + (n, steps, 1e9 * dt * threads /
-                                                                                  (n * steps))))
+                sample="CPU oracle, %dx%dx%d-cell sub-box of the same plasma (%s patches, same species and ppc, "
+                       "%d particles), %d steps, OpenMP over patches; %.1f ns*core/push"
+                       % (shape + ("x".join(map(str, W["patch"])), n, steps, 1e9 * dt * threads / (n * steps))))
 
 
 def main():
@@ -112,6 +147,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="beam", choices=["beam", "thermal"])
+    ap.add_argument("--ppc", type=int, default=8, help="background particles/cell of the beam-plasma workload")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64", help="comma list of mixed,fp64; the first is the headline")
@@ -120,18 +157,20 @@ def main():
         print("bench.py: this build measures one GPU only (the PIC step is not sharded across GPUs here); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    W = workload()
+    W = workload(args.workload, args.ppc)
     forms = [f for f in args.forms.split(",") if f]
     res = {}
     for form in forms:
         g = run_gpu(args, W, form)
         npart, K = g["npart"], args.steps
-        t_part = g["kernel_ms"]["particles"] / 1e3 / K  # s per step, both species
+        t_part = g["kernel_ms"]["particles"] / 1e3 / K  # s per step, all species
         achieved = BYTES_PER_PUSH[form] * npart / t_part / 1e9
+        e0, e1 = np.array(g["energies0"]), np.array(g["energies"])
         res[form] = dict(value=npart * K / g["wall"], ms_per_step=1e3 * g["wall"] / K,
                          ms_per_step_graph=1e3 * g["wall_graph"] / K,
                          kernel_ms_per_step={k: v / K for k, v in g["kernel_ms"].items()},
-                         achieved=achieved, npart=npart, energies=g["energies"])
+                         achieved=achieved, npart=npart, particles_per_species=g["per_species"],
+                         energy_change=float(e1.sum() / e0.sum() - 1), energies=g["energies"])
     head = forms[0]
     h = res[head]
     out = {
@@ -146,10 +185,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if head == "mixed" else "f64",
-        "data": "synthetic (seeded thermal plasma initialised in HBM)",
+        "data": "synthetic (seeded plasma initialised in HBM)",
         "config": {
-            "workload": "3D thermal plasma 256x128x128 cells, dx=0.2, 16^3-cell patches, e-/p+ 8 ppc each "
-                        "(%d particles), order 2, Yee, periodic, atomic tile deposit" % h["npart"],
+            "workload": "%s (%d particles), order 2, Yee, periodic, atomic tile deposit" % (W["desc"], h["npart"]),
             "form": head,
             "particles": h["npart"],
             "forms": {f: {k: v for k, v in r.items() if k != "energies"} for f, r in res.items()},

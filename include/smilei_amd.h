@@ -111,6 +111,16 @@ int smi_set_particles(smi_context *ctx, int species, int64_t n,
  * cells, around the (y,z) axis centre; <= 0 disables) get no particles. */
 int smi_init_species(smi_context *ctx, int species, int ppc, double density,
                      double sigma, double drift_ux, double radius, uint64_t seed);
+/* Profile-driven initialisation, ParticleCreator::create
+ * (src/Particles/ParticleCreator.cpp:151-346): host arrays [ncells] of the
+ * number density and particles-per-cell profiles evaluated at the cell
+ * centres (Profile::valuesAt); a fractional ppc adds one particle on a
+ * regular lattice of cells (:217-231), cells with zero density or ppc get
+ * none (:242-246), weight = density * V_cell / nPart (createWeight :936).
+ * Positions uniform in the cell, Gaussian momenta of std `sigma` plus
+ * `drift` along x. */
+int smi_init_species_profile(smi_context *ctx, int species, const double *density, const double *ppc,
+                             double sigma, double drift_ux, uint64_t seed);
 int64_t smi_num_particles(const smi_context *ctx, int species);
 /* Particles in canonical order (patch-major, then cell key, then arrival
  * order); patch and key receive the owning patch id ((px*npy+py)*npz+pz)

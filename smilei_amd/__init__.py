@@ -86,6 +86,13 @@ class Simulation:
         check(self._L.smi_init_species(self.h, s, int(ppc), float(density), float(sigma), float(drift_ux),
                                        float(radius), int(seed)))
 
+    def init_species_profile(self, s, density, ppc, sigma=0.0, drift_ux=0.0, seed=0):
+        """Per-cell density and particles-per-cell arrays (shape of the grid,
+        values at cell centres), ParticleCreator's rounding and weights."""
+        d = np.ascontiguousarray(np.broadcast_to(density, self.shape), dtype=np.float64)
+        n = np.ascontiguousarray(np.broadcast_to(ppc, self.shape), dtype=np.float64)
+        check(self._L.smi_init_species_profile(self.h, s, _dp(d), _dp(n), float(sigma), float(drift_ux), int(seed)))
+
     def num_particles(self, s):
         return self._L.smi_num_particles(self.h, s)
 

@@ -62,6 +62,14 @@ T *dalloc(int64_t n)
     return (T *)p;
 }
 
+// Temporary device buffer released on scope exit (also on errors).
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    void alloc(int64_t n) { p = dalloc<T>(n); }
+    ~DevBuf() { (void)hipFree(p); }
+};
+
 bool streq(const char *a, const char *b) { return a && b && std::strcmp(a, b) == 0; }
 
 }  // namespace
@@ -424,6 +432,35 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
     SMI_CATCH
 }
 
+namespace {
+
+// Shared tail of the initialisers: draw n particles (particle i in cell
+// cells[i / ppc]), bin and sort them.  cells/weights are device arrays or
+// null; w_max bounds |weight| (fixed-point range of the mixed form).
+void init_particles(smi_context *c, int s, int64_t n, int ppc, const int32_t *d_cells, const double *d_weights,
+                    double weight, double w_max, double sigma, double drift_ux, uint64_t seed)
+{
+    SpeciesState &S = c->sp[s];
+    const Geom &g = c->g;
+    const MixPtrs none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (c->mixed) set_quantum(g, S, w_max);
+    launch_init(g, S.view(c->cur), c->mixed ? S.mview(c->cur) : none, S.keys[c->cur], n, ppc, d_cells, d_weights,
+                weight, sigma, drift_ux, seed * 0x100000001B3ull + (uint64_t)s, c->st);
+    S.n = n;
+    if (!c->mixed) launch_bin_initial(g, S.view(c->cur), n, S.keys[c->cur], c->st);
+    launch_sort(g, S.keys[c->cur], n, S.sb, c->st);
+    SMI_HIP(hipStreamSynchronize(c->st));
+}
+
+double cell_volume_of(const Geom &g)
+{
+    double v = 1.0;
+    for (int d = 0; d < g.nd; ++d) v *= g.dx[d];
+    return v;
+}
+
+}  // namespace
+
 int smi_init_species(smi_context *c, int s, int ppc, double density, double sigma, double drift_ux, double radius,
                      uint64_t seed)
 {
@@ -436,11 +473,10 @@ int smi_init_species(smi_context *c, int s, int ppc, double density, double sigm
     SMI_HIP(hipSetDevice(c->p.device));
     drop_graphs(c);
     SMI_HIP(hipStreamSynchronize(c->st));
-    int32_t *d_cells = nullptr;
+    std::vector<int32_t> act;
     int64_t ncell_act = g.ncells;
     if (radius > 0) {
         // cells whose centre lies within `radius` cells of the transverse axis
-        std::vector<int32_t> act;
         const double cy = 0.5 * g.N[1], cz = 0.5 * g.N[2];
         for (int64_t t = 0; t < g.ncells; ++t) {
             const int k = (int)(t % g.N[2]), j = (int)((t / g.N[2]) % g.N[1]);
@@ -448,26 +484,75 @@ int smi_init_species(smi_context *c, int s, int ppc, double density, double sigm
             if (ry * ry + rz * rz <= radius * radius) act.push_back((int32_t)t);
         }
         ncell_act = (int64_t)act.size();
-        d_cells = dalloc<int32_t>(ncell_act);
-        if (ncell_act)
-            SMI_HIP(hipMemcpy(d_cells, act.data(), act.size() * 4, hipMemcpyHostToDevice));
     }
     const int64_t n = ncell_act * ppc;
-    if (n > S.cap) {
-        (void)hipFree(d_cells);
-        throw Error(SMI_ERR_CAPACITY, "initial particles exceed the species capacity");
+    if (n > S.cap) throw Error(SMI_ERR_CAPACITY, "initial particles exceed the species capacity");
+    DevBuf<int32_t> d_cells;
+    if (radius > 0 && ncell_act) {
+        d_cells.alloc(ncell_act);
+        SMI_HIP(hipMemcpy(d_cells.p, act.data(), act.size() * 4, hipMemcpyHostToDevice));
     }
-    double cell_volume = 1.0;
-    for (int d = 0; d < g.nd; ++d) cell_volume *= g.dx[d];
-    const MixPtrs none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    if (c->mixed) set_quantum(g, S, std::fabs(density * cell_volume / ppc));
-    launch_init(g, S.view(c->cur), c->mixed ? S.mview(c->cur) : none, S.keys[c->cur], n, ppc, d_cells,
-                density * cell_volume / ppc, sigma, drift_ux, seed * 0x100000001B3ull + (uint64_t)s, c->st);
-    S.n = n;
-    if (!c->mixed) launch_bin_initial(g, S.view(c->cur), n, S.keys[c->cur], c->st);
-    launch_sort(g, S.keys[c->cur], n, S.sb, c->st);
+    const double weight = density * cell_volume_of(g) / ppc;
+    init_particles(c, s, n, ppc, radius > 0 ? d_cells.p : nullptr, nullptr, weight, std::fabs(weight), sigma,
+                   drift_ux, seed);
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_init_species_profile(smi_context *c, int s, const double *density, const double *ppc, double sigma,
+                             double drift_ux, uint64_t seed)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
+    if (!density || !ppc) throw Error(SMI_ERR_INVALID, "density and ppc arrays are required");
+    SpeciesState &S = c->sp[s];
+    const Geom &g = c->g;
+    const double vol = cell_volume_of(g);
+    const double inv_nd = 1.0 / g.nd;
+    // number of particles per cell: ParticleCreator.cpp:217-246 (floor, plus
+    // one on a regular lattice of cells for the fractional part; none where
+    // the density vanishes); weight = density * V / nPart (createWeight :936)
+    std::vector<int32_t> cells;
+    std::vector<double> wts;
+    double w_max = 0.;
+    for (int64_t t = 0; t < g.ncells; ++t) {
+        const int64_t idx[3] = {t / ((int64_t)g.N[1] * g.N[2]), (t / g.N[2]) % g.N[1], t % g.N[2]};
+        const double nppc = ppc[t];
+        if (!(nppc >= 0.) || !std::isfinite(nppc)) throw Error(SMI_ERR_INVALID, "ppc must be finite and >= 0");
+        if (!std::isfinite(density[t])) throw Error(SMI_ERR_INVALID, "density must be finite");
+        double np = std::floor(nppc);
+        double ip;
+        if (std::modf(nppc, &ip) > 0) {
+            const double rem = std::pow(nppc - std::floor(nppc), -inv_nd);
+            bool extra = true;
+            for (int d = 0; d < g.nd; ++d) extra = extra && std::fmod((double)idx[d], rem) < 1.;
+            if (extra) np += 1;
+        }
+        double dens = std::fabs(density[t]) < 1e-200 ? 0. : density[t];
+        if (np <= 0 || dens == 0.) continue;
+        const int npart = (int)np;
+        const double w = std::fabs(dens) * vol / npart;
+        w_max = std::max(w_max, w);
+        for (int q = 0; q < npart; ++q) {
+            cells.push_back((int32_t)t);
+            wts.push_back(w);
+        }
+    }
+    const int64_t n = (int64_t)cells.size();
+    if (n > S.cap) throw Error(SMI_ERR_CAPACITY, "initial particles exceed the species capacity");
+    SMI_HIP(hipSetDevice(c->p.device));
+    drop_graphs(c);
     SMI_HIP(hipStreamSynchronize(c->st));
-    (void)hipFree(d_cells);
+    DevBuf<int32_t> d_cells;
+    DevBuf<double> d_w;
+    if (n) {
+        d_cells.alloc(n);
+        d_w.alloc(n);
+        SMI_HIP(hipMemcpy(d_cells.p, cells.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        SMI_HIP(hipMemcpy(d_w.p, wts.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    init_particles(c, s, n, 1, d_cells.p, d_w.p, 0., w_max, sigma, drift_ux, seed);
     return SMI_OK;
     SMI_CATCH
 }

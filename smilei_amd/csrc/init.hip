@@ -1,9 +1,10 @@
 // init.hip -- seeded in-HBM particle initialisation.
 //
 // Plays the role of ParticleCreator (src/Particles/ParticleCreator.cpp) for
-// the synthetic workloads: `ppc` particles per active cell, positions
-// uniform inside the cell, Gaussian momenta, weight density*V_cell/ppc
-// (ParticleCreator.cpp:259).  Random numbers come from a counter-based hash
+// the synthetic workloads: `ppc` particles per active cell (or a per-cell
+// particle list built on the host from density / particles-per-cell
+// profiles), positions uniform inside the cell, Gaussian momenta, weight
+// density*V_cell/ppc (ParticleCreator.cpp:259, createWeight :933-939).  Random numbers come from a counter-based hash
 // (SplitMix64 finaliser of seed, species, particle and draw index), so the
 // result does not depend on the launch shape.
 #include "kernels.h"
@@ -27,11 +28,15 @@ __device__ inline double u01(uint64_t seed, uint64_t i, uint32_t k)
     return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+// Particle i lies in cell cells[i / ppc] (all cells in order when cells is
+// null) and has weight weights[i] (or `weight` when weights is null).
 __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ keys, int64_t n, int ppc,
-                       const int32_t *__restrict__ cells, double weight, double sigma, double drift, uint64_t seed)
+                       const int32_t *__restrict__ cells, const double *__restrict__ weights, double weight,
+                       double sigma, double drift, uint64_t seed)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (weights) weight = weights[i];
     const int64_t c = cells ? (int64_t)cells[i / ppc] : i / ppc;
     const int ck = (int)(c % g.N[2]);
     const int cj = (int)((c / g.N[2]) % g.N[1]);
@@ -85,11 +90,12 @@ __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ key
 }  // namespace
 
 void launch_init(const Geom &g, const PartPtrs &A, const MixPtrs &M, uint32_t *keys, int64_t n, int ppc,
-                 const int32_t *cells, double weight, double sigma, double drift, uint64_t seed, hipStream_t st)
+                 const int32_t *cells, const double *weights, double weight, double sigma, double drift,
+                 uint64_t seed, hipStream_t st)
 {
     if (n == 0) return;
     hipLaunchKernelGGL(k_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, A, M, keys, n, ppc, cells,
-                       weight, sigma, drift, seed);
+                       weights, weight, sigma, drift, seed);
     SMI_LAUNCH_CHECK();
 }
 

@@ -105,7 +105,8 @@ void launch_rho_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &M, cons
 // Writes FP64 particles to A, or (when M.ox != null) mixed particles to M
 // plus their cell keys (direction 13) to `keys`.
 void launch_init(const Geom &g, const PartPtrs &A, const MixPtrs &M, uint32_t *keys, int64_t n, int ppc,
-                 const int32_t *cells, double weight, double sigma, double drift, uint64_t seed, hipStream_t st);
+                 const int32_t *cells, const double *weights, double weight, double sigma, double drift,
+                 uint64_t seed, hipStream_t st);
 
 // ---- fields (fields.hip) ----------------------------------------------
 struct FieldSet {

@@ -10,7 +10,7 @@
 //   2. exclusive scan -> cell_off               (3-phase block scan)
 //   3. scatter of slot indices into buckets     (atomic count-down, which
 //      also returns the histogram to zero for the next step)
-//   4. per-bucket insertion sort by (direction of motion, source slot), the
+//   4. per-bucket rank sort by (direction of motion, source slot), the
 //      canonical order of particles inside a cell.
 // The particle data itself is not moved here: the next step's particle
 // kernel reads its input through `perm`.
@@ -149,13 +149,41 @@ __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const in
 }
 
 // Canonical order inside a cell: by direction of motion, then source slot.
+// One thread per cell.  Buckets of up to kFixRegs particles are ranked in
+// registers (all loads issued together, rank = number of smaller
+// (direction, slot) pairs; the pairs are distinct because slots are);
+// larger buckets fall back to an insertion sort in memory.
+constexpr int kFixRegs = 16;
+
 __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
                              int32_t *__restrict__ perm)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncells) return;
     const int a = cell_off[g], b = cell_off[g + 1];
-    if (b - a < 2) return;
+    const int n = b - a;
+    if (n < 2) return;
+    if (n <= kFixRegs) {
+        int32_t v[kFixRegs];
+        uint64_t k[kFixRegs];
+#pragma unroll
+        for (int i = 0; i < kFixRegs; ++i) v[i] = i < n ? perm[a + i] : 0;
+#pragma unroll
+        for (int i = 0; i < kFixRegs; ++i)
+            k[i] = i < n ? ((uint64_t)(keys[v[i]] >> 27) << 32) | (uint32_t)v[i] : ~0ull;
+        bool sorted = true;
+#pragma unroll
+        for (int i = 1; i < kFixRegs; ++i) sorted &= k[i - 1] < k[i] || i >= n;
+        if (sorted) return;
+#pragma unroll
+        for (int i = 0; i < kFixRegs; ++i) {
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < kFixRegs; ++j) r += k[j] < k[i] ? 1 : 0;
+            if (i < n) perm[a + r] = v[i];
+        }
+        return;
+    }
     for (int i = a + 1; i < b; ++i) {
         const int32_t v = perm[i];
         const uint64_t kv = ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)v;

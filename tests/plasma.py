@@ -7,12 +7,22 @@ uniform-random positions inside each cell, Gaussian momenta of std sigma
 import numpy as np
 
 
-def thermal(shape, dx, ppc, sigma, seed, density=1.0, drift=0.0):
+def thermal(shape, dx, ppc, sigma, seed, density=1.0, drift=0.0, drift_ux=None, radius=0.0):
+    """radius > 0 keeps only the cells whose centre lies within `radius`
+    cells of the x axis (the beam cylinder of the beam-plasma workload)."""
+    if drift_ux is not None:
+        drift = drift_ux
     rng = np.random.default_rng(seed)
     nd = len(shape)
     ncell = int(np.prod(shape))
-    n = ncell * ppc
-    cells = np.repeat(np.arange(ncell), ppc)
+    act = np.arange(ncell)
+    if radius > 0:
+        idx = np.unravel_index(act, shape)
+        ry = idx[1] + 0.5 - 0.5 * shape[1]
+        rz = idx[2] + 0.5 - 0.5 * shape[2] if nd == 3 else 0.0
+        act = act[ry * ry + rz * rz <= radius * radius]
+    n = len(act) * ppc
+    cells = np.repeat(act, ppc)
     idx = np.unravel_index(cells, shape)
     pos = []
     for d in range(3):
