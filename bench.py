@@ -40,6 +40,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic particle bytes per push (read + write of the particle state)
 BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
+# HBM bytes per push of the particle kernels on the beam-plasma workload,
+# measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
+# copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
+# change; None for workloads without a measurement.
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 272.6, "fp64": 353.9}}
 
 
 def cfl_dt(dx, factor=0.95):
@@ -110,6 +115,28 @@ def run_gpu(args, W, form):
                 energies=e.tolist(), energies0=e0.tolist())
 
 
+def copy_probe(nbytes=4 << 30, reps=10):
+    """Device-to-device copy of `nbytes` (hipMemcpyAsync through torch):
+    the HBM bandwidth a plain stream reaches on this box, read + write bytes
+    per second.  Also the known-size kernel that calibrates the FETCH_SIZE /
+    WRITE_SIZE counters."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms / 1e3) / 1e9
+
+
 def run_cpu_baseline(args, W):
     """The oracle on a sub-box of the same plasma (same dx, dt, patch size,
     ppc and species; for the beam-plasma a 128x64x64-cell box with the beam
@@ -152,13 +179,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64", help="comma list of mixed,fp64; the first is the headline")
+    ap.add_argument("--copy-only", action="store_true", help="run the device copy probe only")
     args = ap.parse_args()
+    if args.copy_only:
+        print(json.dumps({"copy_GBs": copy_probe(), "bytes_per_copy": 4 << 30}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the PIC step is not sharded across GPUs here); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     W = workload(args.workload, args.ppc)
     forms = [f for f in args.forms.split(",") if f]
+    copy_gbs = copy_probe()
     res = {}
     for form in forms:
         g = run_gpu(args, W, form)
@@ -173,6 +205,9 @@ def main():
                          energy_change=float(e1.sum() / e0.sum() - 1), energies=g["energies"])
     head = forms[0]
     h = res[head]
+    # HBM bytes per particle-kernel launch (a step launches it once per species)
+    tpp = TRAFFIC_PER_PUSH.get(W["name"], {}).get(head)
+    traffic = tpp * h["npart"] / len(W["species"]) if tpp else None
     out = {
         "metric": "particle-pushes/s (gather+Boris+Esirkepov) 1xMI355X, mixed vs FP64; %HBM peak",
         "value": h["value"],
@@ -198,7 +233,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": h["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_bytes_per_push": tpp,
+            "measured_copy_GBs": copy_gbs,
             "kernel": "%s particle kernel (fused gather+Boris+key+Esirkepov), %d B/push algorithmic"
                       % (head, BYTES_PER_PUSH[head]),
         },
