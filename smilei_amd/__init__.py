@@ -57,6 +57,8 @@ class Simulation:
         check(self._L.smi_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h
         self.ndim = nd
+        self.dx = tuple(float(cl[d]) for d in range(3))
+        self.dt = float(timestep)
         self.shape = tuple(int(nc[d]) if d < nd else 1 for d in range(3))
         self.npatch = int(np.prod([self.shape[d] // int(ps[d]) if d < nd else 1 for d in range(3)]))
         self.species = []

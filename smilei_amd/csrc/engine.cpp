@@ -83,6 +83,7 @@ struct smi_context {
     double *pj = nullptr;          // per-patch ghosted J (deterministic mode)
     DetSpecies *d_det = nullptr;   // device copy of the deposit descriptors
     int d_det_n = 0;
+    MixSpecies *d_mix = nullptr;   // mixed-form particle-launch descriptors [2][species]
     double *partials = nullptr;    // reduction scratch
     double *rho = nullptr;         // diagnostics scratch (ncells)
     int *err = nullptr;            // device flag raised by kernels on corrupt particles
@@ -142,6 +143,28 @@ void upload_det(smi_context *c)
     SMI_HIP(hipMemcpyAsync(c->d_det, v.data(), v.size() * sizeof(DetSpecies), hipMemcpyHostToDevice, c->st));
 }
 
+// Mixed-form launch descriptors for both ping-pong states (set k: input
+// buffer k, output 1-k), refreshed when species change.
+void upload_mix(smi_context *c)
+{
+    if (c->sp.empty() || !c->mixed) return;
+    std::vector<MixSpecies> v;
+    for (int k = 0; k < 2; ++k)
+        for (auto &s : c->sp) {
+            MixSpecies m;
+            m.k = s.km;
+            m.A = s.mview(k);
+            m.B = s.mview(1 - k);
+            m.keysA = s.keys[k];
+            m.keysB = s.keys[1 - k];
+            m.perm = s.sb.perm;
+            m.cell_off = s.sb.cell_off;
+            m.counts = s.sb.counts;
+            v.push_back(m);
+        }
+    SMI_HIP(hipMemcpyAsync(c->d_mix, v.data(), v.size() * sizeof(MixSpecies), hipMemcpyHostToDevice, c->st));
+}
+
 // One PIC time step reading particle buffer c->cur and writing 1 - c->cur.
 void enqueue_step(smi_context *c, bool timed)
 {
@@ -154,11 +177,17 @@ void enqueue_step(smi_context *c, bool timed)
     if (timed) SMI_HIP(hipEventRecord(c->ev[0], c->st));
     if (!det)
         for (int q = 0; q < 3; ++q) launch_fill(f.J[q], g.ncells, 0.0, c->st);  // restartRhoJ
-    for (auto &s : c->sp) {
-        if (c->mixed)
-            launch_particles_mixed(g, s.km, s.mview(a), s.keys[a], s.sb.perm, s.mview(1 - a), s.keys[1 - a],
-                                   s.sb.cell_off, GF, J, c->err, s.sb.counts, c->interleave != 0, c->st);
-        else
+    if (c->mixed) {
+        // species share the cluster's field and J tiles, kMaxMerged per launch
+        const int ns = (int)c->sp.size();
+        for (int g0 = 0; g0 < ns; g0 += kMaxMerged) {
+            const int n = std::min(kMaxMerged, ns - g0);
+            double cr_max = 0.;
+            for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
+            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->interleave != 0, c->st);
+        }
+    } else {
+        for (auto &s : c->sp)
             launch_particles(g, s.k, s.view(a), s.sb.perm, s.view(1 - a), s.keys[1 - a], s.scratch, s.cap,
                              s.sb.cell_off, GF, J, !det, c->err, s.sb.counts, c->st);
     }
@@ -176,8 +205,8 @@ void enqueue_step(smi_context *c, bool timed)
 
 void flip(smi_context *c) { c->cur ^= 1; }
 
-// Fixed-point quantum of the mixed J tile: 2^-32 of the largest current
-// contribution |q| w_max / V * max(dx/dt) of the species.
+// Largest current contribution |q| w_max / V * max(dx/dt) of a species: sets
+// the fixed-point unit of the mixed form's J tile.
 void set_quantum(const Geom &g, SpeciesState &s, double w_max)
 {
     double cell_volume = 1.0, dxdt = 0.;
@@ -187,8 +216,7 @@ void set_quantum(const Geom &g, SpeciesState &s, double w_max)
     }
     double cmax = std::fabs(s.charge) / cell_volume * w_max * dxdt;
     if (!(cmax > 0)) cmax = 1.0;
-    s.km.quantum = std::ldexp(cmax, -32);
-    s.km.inv_quantum = (float)(1.0 / s.km.quantum);
+    s.km.cr_max = cmax;
 }
 
 int fail(const Error &e)
@@ -268,6 +296,10 @@ int smi_create(const smi_params *p, smi_context **out)
         g.xmax[d] = g.dx[d] * (double)g.N[d];
         ncells *= g.N[d];
     }
+    for (int d = 0; d < nd; ++d)
+        if (!(p->timestep < p->cell_length[d]))
+            throw Error(SMI_ERR_INVALID, "timestep must be below every cell_length (c dt < dx): the particle "
+                                         "kernels move a particle by at most one cell per step");
     if (ncells >= (1ll << 27)) throw Error(SMI_ERR_INVALID, "at most 2^27 cells per simulation");
     g.ncells = ncells;
     g.cpp = g.n[0] * g.n[1] * g.n[2];
@@ -321,6 +353,7 @@ int smi_destroy(smi_context *c)
     }
     (void)hipFree(c->pj);
     (void)hipFree(c->d_det);
+    (void)hipFree(c->d_mix);
     (void)hipFree(c->partials);
     (void)hipFree(c->rho);
     (void)hipFree(c->err);
@@ -374,6 +407,12 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     c->d_det_n = (int)c->sp.size();
     c->d_det = dalloc<DetSpecies>(2 * c->d_det_n);
     upload_det(c);
+    (void)hipFree(c->d_mix);
+    c->d_mix = nullptr;
+    if (c->mixed) {
+        c->d_mix = dalloc<MixSpecies>(2 * c->sp.size());
+        upload_mix(c);
+    }
     SMI_HIP(hipStreamSynchronize(c->st));
     if (id) *id = (int)c->sp.size() - 1;
     return SMI_OK;
@@ -686,9 +725,6 @@ int smi_synchronize(smi_context *c)
     SMI_HIP(hipStreamSynchronize(c->st));
     int e = 0;
     SMI_HIP(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
-    if (e & 2)
-        throw Error(SMI_ERR_STATE, "more than 32767 particles in one 4^3-cell cluster: beyond the range of the "
-                                   "mixed form's fixed-point current tile");
     if (e)
         throw Error(SMI_ERR_STATE, "a particle left its cluster stencil (moved more than one cell per step, "
                                    "non-finite position or corrupt input); the step skipped it");

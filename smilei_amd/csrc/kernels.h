@@ -23,8 +23,7 @@ struct SpeciesKM {
     float qmh;          // charge/mass * dt/2
     float cw_scale;     // charge / V_cell
     float dxdt[3];      // dx/dt, dy/dt, dz/dt
-    float inv_quantum;  // 1 / quantum of the fixed-point J tile
-    double quantum;     // 2^-32 * the largest current contribution of the species
+    double cr_max;      // largest current contribution |q| w_max / V * max(dx/dt)
 };
 
 // Fields read by the gather (E and the centred B_m).
@@ -88,10 +87,21 @@ void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *k
 // ---- mixed-precision form (particles_mixed.hip) -------------------------
 // Same contract as launch_particles (always the atomic tile deposit); the
 // cell of particle i is keysA[i], its new key goes to keysB[slot].
-void launch_particles_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &A, const uint32_t *keysA,
-                            const int32_t *perm, const MixPtrs &B, uint32_t *keysB, const int32_t *cell_off,
-                            const GatherFields &F, const JPtrs &J, int *err, int32_t *counts, bool interleave,
-                            hipStream_t st);
+// One species of a merged particle launch of the mixed form (device array).
+constexpr int kMaxMerged = 4;
+struct MixSpecies {
+    SpeciesKM k;
+    MixPtrs A, B;                   // input / output buffers
+    const uint32_t *keysA;          // keys of the input buffer
+    uint32_t *keysB;                // keys written for the output buffer
+    const int32_t *perm, *cell_off;  // sorted view of the input
+    int32_t *counts;                // destination histogram
+};
+// Particle kernel of the mixed form over ns <= kMaxMerged species that share
+// one J tile per cluster; cr_max = the largest current contribution of those
+// species (sets the fixed-point unit of the tile).
+void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
+                            const JPtrs &J, int *err, bool interleave, hipStream_t st);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
 void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm, int64_t n,
                        const PartPtrs &X, hipStream_t st);

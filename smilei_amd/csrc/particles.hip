@@ -30,6 +30,30 @@ namespace {
 
 using namespace dev;
 
+// Esirkepov currents of one 3D particle into the cluster's LDS tile: old
+// delta dold, new position in cell units pnn, old node ip; NT as
+// esirkepov3d_win.
+template <int ORDER, int NT>
+__device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, const double *dold, const double *pnn,
+                                     const int *ip, double cw)
+{
+    using TL = Tile<ORDER>;
+    constexpr int H = TL::H, JT = TL::JT, JT3 = JT * JT * JT;
+    double S0[3][ORDER + 2], DS[3][ORDER + 2];
+    int w0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) esirkepov_window<ORDER>(dold[d], pnn[d], ip[d], S0[d], DS[d], w0[d]);
+    const double cr[3] = {cw * (g.dx[0] / g.dt), cw * (g.dx[1] / g.dt), cw * (g.dx[2] / g.dt)};
+    // tile coords of stencil index 0 along each axis
+    const int o0 = ip[0] - (H + 1) - (G0[0] - TL::JO);
+    const int o1 = ip[1] - (H + 1) - (G0[1] - TL::JO);
+    const int o2 = ip[2] - (H + 1) - (G0[2] - TL::JO);
+    double *sJb = sJ + (o0 * JT + o1) * JT + o2;
+    esirkepov3d_win<ORDER, NT>(S0, DS, w0, cr, [&](int c, int a, int b, int k, double v) {
+        atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
+    });
+}
+
 // ---------------------------------------------------------------------------
 // Fused particle kernel.
 // ---------------------------------------------------------------------------
@@ -47,6 +71,14 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     __shared__ double sF[6 * GT3];
     __shared__ double sJ[DEPOSIT ? 3 * JT3 : 1];
     __shared__ int sHist[64];  // destination-cell counts of this cluster's own cells
+    // 3D deposit: particles whose node changes on some axis are deposited in
+    // a second pass over this queue (old delta, new position in cells, charge
+    // weight, old node relative to the cluster), so that the main loop runs
+    // the narrow stencil with all lanes
+    constexpr bool QUEUE = DEPOSIT && ND == 3;
+    constexpr int QCAP = QUEUE ? 128 : 1;
+    __shared__ double sQ[QCAP][8];
+    __shared__ int sQn;
 
     const Cluster cl = decode_cluster(g, blockIdx.x);
     const int s = cell_off[cl.gbase], e = cell_off[cl.gbase + g.cells_per_cl];
@@ -75,6 +107,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     if constexpr (DEPOSIT)
         for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.;
     if (threadIdx.x < 64) sHist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
 
     double pmin[3], pmax[3];
@@ -203,22 +236,37 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
             // ---- Esirkepov currents into the LDS tile
             const double cw = sp.cw_scale * w;
             if constexpr (ND == 3) {
-                double S0[3][ORDER + 2], DS[3][ORDER + 2];
-                int w0[3], sh[3];
+                double pnn[3];
+                int moved = 0;
 #pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    esirkepov_window<ORDER>(dold[d], xn[d] * g.dinv[d], ip[d], S0[d], DS[d], w0[d], &sh[d]);
-                const double cr[3] = {cw * (g.dx[0] / g.dt), cw * (g.dx[1] / g.dt), cw * (g.dx[2] / g.dt)};
-                // tile coords of stencil index 0 along each axis
-                const int o0 = ip[0] - (H + 1) - (cl.G0[0] - TL::JO);
-                const int o1 = ip[1] - (H + 1) - (cl.G0[1] - TL::JO);
-                const int o2 = ip[2] - (H + 1) - (cl.G0[2] - TL::JO);
-                double *sJb = sJ + (o0 * JT + o1) * JT + o2;
-                const auto tile_add = [&](int c, int a, int b, int k, double v) {
-                    atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
-                };
-                if ((sh[0] | sh[1] | sh[2]) == 0) esirkepov3d_win<ORDER, ORDER + 1>(S0, DS, w0, cr, tile_add);
-                else esirkepov3d_win<ORDER>(S0, DS, w0, cr, tile_add);
+                for (int d = 0; d < 3; ++d) {
+                    pnn[d] = xn[d] * g.dinv[d];
+                    moved |= (int)round(pnn[d]) - ip[d];
+                }
+                if (moved == 0) deposit3_tile<ORDER, ORDER + 1>(sJ, g, cl.G0, dold, pnn, ip, cw);
+                const uint64_t gm = __ballot(moved != 0);
+                if (gm) {
+                    const int lane = threadIdx.x & 63;
+                    const int leader = __ffsll((unsigned long long)gm) - 1;
+                    int base = 0;
+                    if (lane == leader) base = atomicAdd(&sQn, __popcll(gm));
+                    base = __shfl(base, leader, 64);
+                    if (moved != 0) {
+                        const int pos = base + __popcll(gm & ((1ull << lane) - 1));
+                        if (pos < QCAP) {
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) {
+                                sQ[pos][d] = dold[d];
+                                sQ[pos][3 + d] = pnn[d];
+                            }
+                            sQ[pos][6] = cw;
+                            sQ[pos][7] = (double)((ip[0] - cl.G0[0]) | ((ip[1] - cl.G0[1]) << 4) |
+                                                  ((ip[2] - cl.G0[2]) << 8));
+                        } else {
+                            deposit3_tile<ORDER, ORDER + 2>(sJ, g, cl.G0, dold, pnn, ip, cw);
+                        }
+                    }
+                }
             } else {
                 double S0[3][7], S1[3][7], DS[3][7];
 #pragma unroll
@@ -239,6 +287,18 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     if (threadIdx.x < g.cells_per_cl) {
         const int c = sHist[threadIdx.x];
         if (c) atomicAdd(&counts[cl.gbase + threadIdx.x], c);
+    }
+    if constexpr (QUEUE) {
+        const int nq = sQn < QCAP ? sQn : QCAP;
+        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+            const double dold[3] = {sQ[t][0], sQ[t][1], sQ[t][2]};
+            const double pnn[3] = {sQ[t][3], sQ[t][4], sQ[t][5]};
+            const int packed = (int)sQ[t][7];
+            const int ip[3] = {cl.G0[0] + (packed & 15), cl.G0[1] + ((packed >> 4) & 15),
+                               cl.G0[2] + ((packed >> 8) & 15)};
+            deposit3_tile<ORDER, ORDER + 2>(sJ, g, cl.G0, dold, pnn, ip, sQ[t][6]);
+        }
+        __syncthreads();
     }
     if constexpr (DEPOSIT) {
         double *dstJ[3] = {J.Jx, J.Jy, J.Jz};

@@ -91,17 +91,65 @@ __device__ inline void windowf(float dold, float dnew, int sh, float *S0w, float
     }
 }
 
-// Fixed-point accumulation of the J tile.  A contribution c, already
-// scaled to units of the quantum 2^-32 * cr_max of its species, is rounded
-// to an integer k by one FP64 add of 1.5*2^52 (the sum's mantissa then holds
-// k); the raw bits of that double, 0x4338000000000000 + k, are summed with
-// an integer LDS atomic.  The constant only reaches bits 48..63, so the low
-// 48 bits of a tile word hold sum(k) exactly while |sum(k)| < 2^47, i.e.
-// for fewer than 2^15 contributions per point (|k| <= 2^32 because
-// |c| <= cr).  Integer adds make the tile sums independent of the order of
-// the atomics; the quantum is 2.3e-10 of the largest contribution.
+// FP64 B-spline weights (Horner form) of the deposit.  The deposit is done
+// in FP64 from the FP32 in-cell deltas: with FP32 weights the Esirkepov
+// identity sum_c DS_c T_c = S1S1S1 - S0S0S0 holds only to FP32 rounding per
+// particle and step, and the Gauss-law residual random-walks (measured:
+// 1.3e-6 of max|rho| after 1000 steps at order 4); in FP64 it holds to
+// 1e-16, so div E - rho stays at its initial value.  FP64 VALU issue rate
+// equals the FP32 one on gfx950.
+template <int ORDER>
+__device__ inline void splined(double d, double *w)
+{
+    if constexpr (ORDER == 2) {
+        w[0] = 0.5 * fma(d, d - 1.0, 0.25);
+        w[1] = fma(-d, d, 0.75);
+        w[2] = 0.5 * fma(d, d + 1.0, 0.25);
+    } else {
+        const double d2 = d * d;
+        const double a = 0.5 - d, b = 0.5 + d;
+        const double a2 = a * a, b2 = b * b;
+        w[0] = a2 * a2 * (1.0 / 24.0);
+        w[4] = b2 * b2 * (1.0 / 24.0);
+        w[2] = fma(d2, fma(d2, 0.25, -0.625), 115.0 / 192.0);
+        const double c = fma(d2, -(1.0 / 6.0), 0.25);
+        w[1] = fma(d, fma(d, c, 1.0 / 6.0 * d2 - 11.0 / 24.0), 19.0 / 96.0);
+        w[3] = fma(d, fma(d, c, -(1.0 / 6.0) * d2 + 11.0 / 24.0), 19.0 / 96.0);
+    }
+}
+
+// Esirkepov window in FP64 (see esirkepov_window): S0 from the old delta,
+// S1 from the new delta shifted by sh in {-1,0,1}.
+template <int ORDER>
+__device__ inline void windowd(double dold, double dnew, int sh, double *S0w, double *DSw, int &w0)
+{
+    constexpr int W = ORDER + 2;
+    double s[ORDER + 1], nw[ORDER + 1];
+    splined<ORDER>(dold, s);
+    splined<ORDER>(dnew, nw);
+    w0 = sh < 0 ? 0 : 1;
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        const double a = w0 == 0 ? (m == 0 ? 0. : s[m - 1]) : (m <= ORDER ? s[m] : 0.);
+        const double b = sh <= 0 ? (m <= ORDER ? nw[m] : 0.) : (m >= 1 ? nw[m - 1] : 0.);
+        S0w[m] = a;
+        DSw[m] = b - a;
+    }
+}
+
+// Fixed-point accumulation of the J tile.  A contribution c, scaled to
+// units of the cluster's quantum, is rounded to an integer k by one FP64 add
+// of 1.5*2^52 (the sum's mantissa then holds k); the raw bits of that
+// double, 0x4338000000000000 + k, are summed with an integer LDS atomic.
+// The constant only reaches bits 48..63, so the low 48 bits of a tile word
+// hold sum(k) exactly while |sum(k)| < 2^47.  The quantum of a cluster with
+// N particles is 2^-46 * N * cr_max (cr_max = the largest contribution of
+// any species of the launch): every particle adds at most once to a tile
+// word and |k| <= 2^46 / N, so the sum stays in range for any N, and the
+// quantum is 2^-36 of cr_max for a 1024-particle cluster.  Integer adds make
+// the tile sums exact and independent of the order of the atomics.
 constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
-constexpr int kMaxClusterParticles = 32767;
+constexpr double kUnit = 1.4210854715202004e-14;  // 2^-46
 
 __device__ inline double fixed_decode(unsigned long long v, double quantum)
 {
@@ -110,44 +158,38 @@ __device__ inline double fixed_decode(unsigned long long v, double quantum)
 }
 
 // Windowed 3D Esirkepov with W(b,c) = S0b (S0c + DSc/2) + DSb (S0c/2 + DSc/3)
-// (the reference's four-term weight, factorised), FP32 weights, FP64 prefix
+// (the reference's four-term weight, factorised), FP64 weights and prefix
 // sums along the current's axis, fixed-point tile.  Only the non-zero part
 // of the stencil is visited: NT window entries across the current (entries
 // past the union of the old and new supports are exact zeros) and NA faces
 // along it.  The face after the last window entry is left out: its prefix
-// sum is sum(DS) over the whole window, zero up to the FP32 rounding of the
-// shape factors.  A particle whose nearest node is unchanged on every axis
-// has NT = ORDER+1, NA = ORDER (the two supports coincide); otherwise
-// NT = ORDER+2, NA = ORDER+1.  tile(c, i, j, k) gives the LDS word of
-// component c at window indices relative to the particle's stencil origin.
+// sum is sum(DS) over the whole window, zero up to FP64 rounding.  A
+// particle whose nearest node is unchanged on every axis has NT = ORDER+1,
+// NA = ORDER (the two supports coincide); otherwise NT = ORDER+2,
+// NA = ORDER+1.  tile(c, i, j, k) gives the LDS word of component c at
+// window indices relative to the particle's stencil origin.
 template <int ORDER, int NT, int NA, class Tile>
-__device__ inline void esirkepov3_fixed(const float (&S0)[3][ORDER + 2], const float (&DS)[3][ORDER + 2],
-                                        const int *w0, const float *crq, Tile tile)
+__device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const double (&DS)[3][ORDER + 2],
+                                        const int *w0, const double *crq, Tile tile)
 {
-    float P[3][NT], Q[3][NT];
-    double DSd[3][NA];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int m = 0; m < NT; ++m) {
-            P[d][m] = fmaf(0.5f, DS[d][m], S0[d][m]);
-            Q[d][m] = fmaf(1.f / 3.f, DS[d][m], 0.5f * S0[d][m]);
-        }
-#pragma unroll
-        for (int m = 0; m < NA; ++m) DSd[d][m] = (double)DS[d][m];
-    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int bx = (c + 1) % 3, ex = (c + 2) % 3;
+        double P[NT], Q[NT];
+#pragma unroll
+        for (int m = 0; m < NT; ++m) {
+            P[m] = fma(0.5, DS[ex][m], S0[ex][m]);
+            Q[m] = fma(1.0 / 3.0, DS[ex][m], 0.5 * S0[ex][m]);
+        }
 #pragma unroll
         for (int a = 0; a < NT; ++a)
 #pragma unroll
             for (int b = 0; b < NT; ++b) {
-                const double t = (double)(crq[c] * fmaf(S0[bx][a], P[ex][b], DS[bx][a] * Q[ex][b]));
+                const double t = crq[c] * fma(S0[bx][a], P[b], DS[bx][a] * Q[b]);
                 double acc = 0.;
 #pragma unroll
                 for (int r = 1; r <= NA; ++r) {
-                    acc = fma(-DSd[c][r - 1], t, acc);
+                    acc = fma(-DS[c][r - 1], t, acc);
                     int idx[3];
                     idx[c] = w0[c] + r;
                     idx[bx] = w0[bx] + a;
@@ -157,6 +199,24 @@ __device__ inline void esirkepov3_fixed(const float (&S0)[3][ORDER + 2], const f
                 }
             }
     }
+}
+
+// Esirkepov deposit of one particle from its old and new in-cell deltas
+// (dold, dnew in [-1/2, 1/2)), node shifts sh and the tile coordinates o3 of
+// its stencil origin; NT/NA as esirkepov3_fixed.
+template <int ORDER, int NT, int NA, int JT>
+__device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, const float *dnew, const int *sh,
+                                     const int *o3, const double *crq)
+{
+    constexpr int JT3 = JT * JT * JT;
+    double S0[3][ORDER + 2], DS[3][ORDER + 2];
+    int w0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) windowd<ORDER>((double)dold[d], (double)dnew[d], sh[d], S0[d], DS[d], w0[d]);
+    unsigned long long *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
+    esirkepov3_fixed<ORDER, NT, NA>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
+        return &sJb[c * JT3 + (a * JT + b) * JT + k];
+    });
 }
 
 struct Cl {
@@ -191,11 +251,12 @@ __device__ inline uint32_t global_key3(const Geom &g, const int *c, int *pc)
     return (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
 }
 
+// One workgroup per 4^3 cluster runs the particles of every species of the
+// launch (specs[0..ns)) against one field tile and one J tile.
 template <int ORDER, bool INTERLEAVE>
-__global__ void __launch_bounds__(256, ORDER == 2 ? 4 : 2)
-k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ keysA, const int32_t *__restrict__ perm,
-                  MixPtrs B, uint32_t *__restrict__ keysB, const int32_t *__restrict__ cell_off, GatherFields F,
-                  JPtrs J, int *__restrict__ err, int32_t *__restrict__ counts)
+__global__ void __launch_bounds__(256, ORDER == 2 ? 3 : 2)
+k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
+                  int *__restrict__ err)
 {
     constexpr int H = ORDER / 2;
     // J tile: stencil rows [c0-H-1, c0+4+H]
@@ -203,11 +264,20 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
     constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
     __shared__ float sF[6 * GT3];
     __shared__ unsigned long long sJ[3 * JT3];
-    __shared__ int sHist[64];
+    __shared__ int sHist[kMaxMerged][64];
+    // particles whose node changes on some axis: their deposit (the wider
+    // stencil) is deferred to a second pass over this queue, so that the
+    // wavefronts of the main loop run the narrow stencil with all lanes;
+    // entry = old and new deltas, charge weight in quanta, packed stencil
+    // origin and shifts
+    constexpr int QCAP = ORDER == 2 ? 256 : 128;
+    __shared__ float sQ[QCAP][8];
+    __shared__ int sQn;
 
     const Cl cl = cluster_of(g, blockIdx.x);
-    const int s = cell_off[cl.gbase], e = cell_off[cl.gbase + 64];
-    if (s == e) return;
+    int total = 0;
+    for (int q = 0; q < ns; ++q) total += specs[q].cell_off[cl.gbase + 64] - specs[q].cell_off[cl.gbase];
+    if (total == 0) return;
 
     const double *src[6] = {F.Ex, F.Ey, F.Ez, F.Bx, F.By, F.Bz};
     for (int t = threadIdx.x; t < GT3; t += blockDim.x) {
@@ -218,11 +288,24 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
         for (int f = 0; f < 6; ++f) sF[f * GT3 + t] = (float)src[f][idx];
     }
     for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0ull;
-    if (threadIdx.x < 64) sHist[threadIdx.x] = 0;
-    if (threadIdx.x == 0 && e - s > kMaxClusterParticles) atomicOr(err, 2);  // fixed-point headroom
+    for (int t = threadIdx.x; t < kMaxMerged * 64; t += blockDim.x) sHist[t / 64][t % 64] = 0;
+    if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
 
+    const double quantum = kUnit * (double)total * cr_max, inv_quantum = 1.0 / quantum;
     const float dtdx[3] = {(float)(g.dt * g.dinv[0]), (float)(g.dt * g.dinv[1]), (float)(g.dt * g.dinv[2])};
+    // cr_c = q w / V * dx_c / dt with one common factor: the ratio between
+    // components must be exact for the discrete continuity equation
+    const double dxdt[3] = {g.dx[0] / g.dt, g.dx[1] / g.dt, g.dx[2] / g.dt};
+    for (int sid = 0; sid < ns; ++sid) {
+    const MixSpecies &S = specs[sid];
+    const SpeciesKM &sp = S.k;
+    const MixPtrs A = S.A, B = S.B;
+    const uint32_t *__restrict__ keysA = S.keysA;
+    uint32_t *__restrict__ keysB = S.keysB;
+    const int32_t *__restrict__ perm = S.perm;
+    int32_t *__restrict__ counts = S.counts;
+    const int s = S.cell_off[cl.gbase], e = S.cell_off[cl.gbase + 64];
     const int N = e - s;
     const int q = INTERLEAVE ? (N + 63) >> 6 : 1;
     const int span = INTERLEAVE ? 64 * q : N;
@@ -241,7 +324,7 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
             B.ox[jslot] = o[0]; B.oy[jslot] = o[1]; B.oz[jslot] = o[2];
             B.ux[jslot] = u[0]; B.uy[jslot] = u[1]; B.uz[jslot] = u[2];
             B.w[jslot] = w;
-            atomicAdd(&sHist[0], 1);
+            atomicAdd(&sHist[sid][0], 1);
             continue;
         }
         const int l[3] = {lc >> 4, (lc >> 2) & 3, lc & 3};
@@ -316,23 +399,46 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
 
         // ---- Esirkepov deposit into the FP32 LDS tile
         {
-            float S0[3][ORDER + 2], DS[3][ORDER + 2];
-            int w0[3], o3[3], moved = 0;
+            int sh[3], o3[3], moved = 0;
+            float dnew[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int hn = on[d] >= 0.5f ? 1 : 0;
-                const int sh = (cn[d] - cl.G0[d] - l[d]) + hn - hi[d];  // new node - old node
-                moved |= sh;
-                windowf<ORDER>(dold[d], on[d] - (float)hn, sh, S0[d], DS[d], w0[d]);
+                sh[d] = (cn[d] - cl.G0[d] - l[d]) + hn - hi[d];  // new node - old node
+                moved |= sh[d];
+                dnew[d] = on[d] - (float)hn;
                 o3[d] = l[d] + hi[d];  // old node relative to G0 == tile coordinate of stencil index 0 (JO = H+1)
             }
-            // contributions in units of the species' fixed-point quantum
-            const float cwq = sp.cw_scale * w * sp.inv_quantum;
-            const float crq[3] = {cwq * sp.dxdt[0], cwq * sp.dxdt[1], cwq * sp.dxdt[2]};
-            unsigned long long *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
-            const auto tile = [&](int c, int a, int b, int k) { return &sJb[c * JT3 + (a * JT + b) * JT + k]; };
-            if (moved == 0) esirkepov3_fixed<ORDER, ORDER + 1, ORDER>(S0, DS, w0, crq, tile);
-            else esirkepov3_fixed<ORDER, ORDER + 2, ORDER + 1>(S0, DS, w0, crq, tile);
+            // contributions in units of the cluster's fixed-point quantum
+            const double cwq = (double)(sp.cw_scale * w) * inv_quantum;
+            const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
+            if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER, JT>(sJ, dold, dnew, sh, o3, crq);
+            // node changers: one queue slot range per wavefront
+            const uint64_t gm = __ballot(moved != 0);
+            if (gm) {
+                const int lane = threadIdx.x & 63;
+                const int leader = __ffsll((unsigned long long)gm) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&sQn, __popcll(gm));
+                base = __shfl(base, leader, 64);
+                if (moved != 0) {
+                    const int pos = base + __popcll(gm & ((1ull << lane) - 1));
+                    if (pos < QCAP) {
+                        const int packed = o3[0] | (o3[1] << 3) | (o3[2] << 6) | ((sh[0] + 1) << 9) |
+                                           ((sh[1] + 1) << 11) | ((sh[2] + 1) << 13);
+                        sQ[pos][0] = dold[0];
+                        sQ[pos][1] = dold[1];
+                        sQ[pos][2] = dold[2];
+                        sQ[pos][3] = dnew[0];
+                        sQ[pos][4] = dnew[1];
+                        sQ[pos][5] = dnew[2];
+                        sQ[pos][6] = (float)(sp.cw_scale * w);
+                        sQ[pos][7] = __int_as_float(packed);
+                    } else {
+                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JT>(sJ, dold, dnew, sh, o3, crq);
+                    }
+                }
+            }
         }
 
         // ---- destination key, direction of motion, store
@@ -348,17 +454,30 @@ k_particles_mixed(Geom g, SpeciesKM sp, MixPtrs A, const uint32_t *__restrict__ 
         B.uz[jslot] = un[2];
         B.w[jslot] = w;
         const int64_t rc = (int64_t)gkey - cl.gbase;
-        if (rc >= 0 && rc < 64) atomicAdd(&sHist[rc], 1);
+        if (rc >= 0 && rc < 64) atomicAdd(&sHist[sid][rc], 1);
         else atomicAdd(&counts[gkey], 1);
     }
+    }  // species
 
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int c = sHist[threadIdx.x];
-        if (c) atomicAdd(&counts[cl.gbase + threadIdx.x], c);
+    for (int t = threadIdx.x; t < ns * 64; t += blockDim.x) {
+        const int c = sHist[t / 64][t % 64];
+        if (c) atomicAdd(&specs[t / 64].counts[cl.gbase + t % 64], c);
     }
+    // deferred deposits of the node changers
+    const int nq = sQn < QCAP ? sQn : QCAP;
+    for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+        const float dold[3] = {sQ[t][0], sQ[t][1], sQ[t][2]};
+        const float dnew[3] = {sQ[t][3], sQ[t][4], sQ[t][5]};
+        const double cwq = (double)sQ[t][6] * inv_quantum;
+        const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
+        const int packed = __float_as_int(sQ[t][7]);
+        const int o3[3] = {packed & 7, (packed >> 3) & 7, (packed >> 6) & 7};
+        const int sh[3] = {((packed >> 9) & 3) - 1, ((packed >> 11) & 3) - 1, ((packed >> 13) & 3) - 1};
+        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JT>(sJ, dold, dnew, sh, o3, crq);
+    }
+    __syncthreads();
     double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
-    const double quantum = sp.quantum;
     for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
         const double v = fixed_decode(sJ[t], quantum);
         if (v == 0.) continue;
@@ -487,15 +606,13 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
-void launch_particles_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &A, const uint32_t *keysA,
-                            const int32_t *perm, const MixPtrs &B, uint32_t *keysB, const int32_t *cell_off,
-                            const GatherFields &F, const JPtrs &J, int *err, int32_t *counts, bool interleave,
-                            hipStream_t st)
+void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
+                            const JPtrs &J, int *err, bool interleave, hipStream_t st)
 {
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
+    if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
     const dim3 grid(g.npatches * g.clpp), block(256);
-#define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), grid, block, 0, st, g, sp, A, keysA, perm, B, keysB, \
-                                        cell_off, F, J, err, counts)
+#define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), grid, block, 0, st, g, specs, ns, cr_max, F, J, err)
     if (g.order == 2) {
         if (interleave) SMI_PM(2, true);
         else SMI_PM(2, false);

@@ -1,0 +1,135 @@
+"""The workload configurations end to end on the GPU engine.
+
+* 2D3V thermal plasma (256x256 cells, 32x32 patches, 16 ppc, sigma 0.1,
+  immobile neutralising ions, 100 steps): FP64 deterministic form bit-exact
+  against the oracle, atomic form within 1e-12 of it.
+* LWFA-like uneven patches (reduced 256x32x32 box of the 512x128x128 setup:
+  density ramp + channel, 0.5 ppc, laser initial field): the deterministic
+  FP64 form bit-exact against the oracle from the same particles and fields;
+  the mixed form against the FP64 form.
+* 4th-order mixed form against the FP64 form from identical seeded initial
+  conditions over 1000 steps (reduced 48^3 box of the 3D thermal plasma).
+
+Error metrics of the mixed form: relative field and kinetic energy
+differences against the FP64 run and the Gauss-law drift
+|(div E - rho)(t) - (div E - rho)(0)|_max / max|rho(0)|.
+"""
+import numpy as np
+import pytest
+
+import smilei_amd as sa
+from oracle import pyoracle as O
+from tests.plasma import cfl_dt, load, lwfa, thermal
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz")
+
+
+def copy_to_oracle(sim, nspecies, masses, charges, o):
+    for s in range(nspecies):
+        p = sim.get_particles(s)
+        o.set_particles(o.add_species(masses[s], charges[s]), p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"],
+                        p["w"])
+    for f in FIELDS:
+        o.set_field(f, sim.get_field(f))
+
+
+def compare(g, o, nspecies, exact, tol=0.0):
+    for s in range(nspecies):
+        a, b = g.get_particles(s), o.get_particles(s)
+        np.testing.assert_array_equal(a["patch"], b["patch"])
+        np.testing.assert_array_equal(a["key"], b["key"])
+        np.testing.assert_array_equal(g.patch_counts(s), o.patch_counts(s))
+        for k in ("x", "y", "z", "px", "py", "pz", "w"):
+            if exact:
+                np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+            else:
+                scale = max(np.abs(b[k]).max(), 1e-300)
+                assert np.abs(a[k] - b[k]).max() <= tol * scale, k
+    for f in FIELDS:
+        ref = o.get_field(f)
+        if exact:
+            np.testing.assert_array_equal(g.get_field(f), ref, err_msg=f)
+        else:
+            assert np.abs(g.get_field(f) - ref).max() <= tol * max(np.abs(ref).max(), 1e-300), f
+
+
+def plasma2d(deposit):
+    shape, patch, dx = (256, 256), (32, 32), (0.125, 0.125)
+    dt = cfl_dt(dx)
+    g = sa.Simulation("2Dcartesian", 2, shape, patch, dx, dt, deposit=deposit)
+    o = O.OracleSim(2, 2, shape, patch, dx, dt)
+    p = thermal(shape, dx, 16, 0.1, 11)
+    load(g, g.add_species("electron", 1.0, -1.0, len(p["x"])), p)
+    o.set_particles(o.add_species(1.0, -1.0), p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    return g, o
+
+
+def test_thermal2d_full_size_deterministic_bit_exact():
+    g, o = plasma2d(sa.DEPOSIT_DETERMINISTIC)
+    g.step(100)
+    o.step(100)
+    compare(g, o, 1, exact=True)
+    # energies are reductions: summation order differs from the oracle's
+    np.testing.assert_allclose(g.energies(), o.energies(), rtol=1e-12)
+
+
+def test_thermal2d_full_size_atomic_within_1e12():
+    g, o = plasma2d(sa.DEPOSIT_ATOMIC)
+    g.step(100)
+    o.step(100)
+    compare(g, o, 1, exact=False, tol=1e-12)
+
+
+def test_lwfa_deterministic_bit_exact_vs_oracle():
+    shape, patch, dx = (256, 32, 32), (32, 16, 16), (0.1, 0.5, 0.5)
+    g = lwfa(sa.Simulation, sa.PLAIN_FP64, shape, patch, dx, deposit=sa.DEPOSIT_DETERMINISTIC)
+    counts = g.patch_counts(0)
+    assert counts.min() == 0 and counts.max() == 4096  # uneven: empty and full patches
+    o = O.OracleSim(3, 2, shape, patch, dx, cfl_dt(dx))
+    copy_to_oracle(g, 1, [1.0], [-1.0], o)
+    g.step(20)
+    o.step(20)
+    compare(g, o, 1, exact=True)
+
+
+def errors(m, f, gm0, rho0):
+    em, ef = m.energies(), f.energies()
+    tot = ef.sum()
+    return dict(field=abs(em[0] - ef[0]) / tot, kinetic=abs(em[1:].sum() - ef[1:].sum()) / tot,
+                total=abs(em.sum() - tot) / tot, gauss=np.abs(m.gauss() - gm0).max() / rho0)
+
+
+def test_lwfa_mixed_vs_fp64():
+    shape, patch, dx = (256, 32, 32), (32, 16, 16), (0.1, 0.5, 0.5)
+    f = lwfa(sa.Simulation, sa.PLAIN_FP64, shape, patch, dx)
+    m = lwfa(sa.Simulation, sa.MIXED, shape, patch, dx)
+    gm0, rho0 = m.gauss(), np.abs(m.rho()).max()
+    f.step(200)
+    m.step(200)
+    err = errors(m, f, gm0, rho0)
+    print("lwfa mixed vs fp64 after 200 steps:", err)
+    assert err["field"] < 1e-5 and err["kinetic"] < 1e-5, err
+    assert err["gauss"] < 1e-6, err
+
+
+def test_order4_mixed_vs_fp64_1000_steps():
+    shape, patch, dx = (48, 48, 48), (16, 16, 16), (0.2, 0.2, 0.2)
+    dt = cfl_dt(dx)
+    sims = []
+    for prec in (sa.PLAIN_FP64, sa.MIXED):
+        sim = sa.Simulation("3Dcartesian", 4, shape, patch, dx, dt, precision=prec)
+        for k, (name, mass, q, sig) in enumerate([("electron", 1.0, -1.0, 0.05),
+                                                   ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))]):
+            p = thermal(shape, dx, 8, sig, 40 + k)
+            load(sim, sim.add_species(name, mass, q, len(p["x"])), p)
+        sims.append(sim)
+    f, m = sims
+    gm0, rho0 = m.gauss(), np.abs(m.rho()).max()
+    f.step(1000)
+    m.step(1000)
+    err = errors(m, f, gm0, rho0)
+    print("order-4 mixed vs fp64 after 1000 steps:", err)
+    assert err["field"] < 1e-5 and err["kinetic"] < 1e-5 and err["total"] < 1e-5, err
+    assert err["gauss"] < 1e-6, err

@@ -55,6 +55,7 @@ def _params(**kw):
     (dict(n_cells=(40, 32, 32)), "divide"),
     (dict(patch_size=(4, 16, 16), n_cells=(32, 32, 32)), "oversize"),
     (dict(timestep=0.0), "timestep"),
+    (dict(timestep=0.2), "one cell per step"),
 ])
 def test_factories_reject_unsupported_namelist_values(kw, msg):
     """Like the reference factories, unsupported choices are errors; the
