@@ -145,12 +145,33 @@ int smi_gauss(smi_context *ctx, double *global);
 int smi_profile_step(smi_context *ctx, float *ms);
 /* The stream all work of the context is enqueued on (hipStream_t). */
 void *smi_stream(smi_context *ctx);
-/* Launch-shape knobs searched by the autotuner (no effect on results beyond
- * the order of the atomic J summation).  SMI_TUNE_INTERLEAVE: 1 = lane l of
- * a wavefront takes the l-th slice of its cluster's cell-sorted particles
- * (lanes in distinct cells), 0 = consecutive particles. */
+/* Launch-shape knobs of the particle kernels, searched by smi_autotune (no
+ * effect on results beyond the order of the atomic J summation).
+ * SMI_TUNE_INTERLEAVE (mixed form): 1 = lane l of a wavefront takes the l-th
+ * slice of its cluster's cell-sorted particles (lanes in distinct cells),
+ * 0 = consecutive particles.  SMI_TUNE_THREADS: workgroup size per 4^3
+ * cluster, 64 | 128 | 256 | 512 (particles per thread = cluster particles /
+ * threads).  SMI_TUNE_PERSISTENT (mixed form): 0 = one workgroup per
+ * cluster of the grid, 1 = a grid of the resident workgroups pulling the
+ * non-empty clusters from a work list built each step. */
 #define SMI_TUNE_INTERLEAVE 1
+#define SMI_TUNE_THREADS 2
+#define SMI_TUNE_PERSISTENT 3
+/* Not searched by the autotuner (it changes what smi_get_particles returns):
+ * SMI_TUNE_CELL_ORDER 1 (default) keeps the canonical order of particles
+ * inside a cell (direction of motion, then previous slot); 0 leaves it to
+ * the atomic scatter, like the reference's GPU sort (thrust sort_by_key,
+ * nvidiaParticles.cu:1217-1247), and saves the in-cell sort pass.  The
+ * deterministic deposit always keeps the canonical order. */
+#define SMI_TUNE_CELL_ORDER 4
 int smi_set_tuning(smi_context *ctx, int key, int value);
+int smi_get_tuning(smi_context *ctx, int key, int *value);
+/* Project autotuner (no counterpart in the reference, whose GPU launch shapes
+ * are fixed): times the particle-kernel launches of one step on the live
+ * particle state for every lane mapping x launch form (mixed form) x
+ * workgroup size 128/256/512, `reps` launches each, and keeps the fastest; best_ms (may be
+ * NULL) receives its time.  Leaves the simulation state as it was. */
+int smi_autotune(smi_context *ctx, int reps, float *best_ms);
 
 /* ---- per-operator entry points (one patch, reference layout) -------- *
  * Field arrays are the patch's ghosted arrays: primal extent

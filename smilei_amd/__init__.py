@@ -2,8 +2,9 @@
 
 Python is only the driver: the particle loop, the sort/exchange and the Yee
 solver run in libsmilei_amd.so (HIP, gfx950).  ``Simulation`` wraps one
-context of the C ABI (include/smilei_amd.h); ``namelist`` builds one from
-Smilei-style ``Main`` / ``Species`` blocks.
+context of the C ABI (include/smilei_amd.h); ``profiles`` evaluates
+namelist-style density / particles-per-cell profiles and external fields on
+the grid.
 """
 import ctypes
 
@@ -12,10 +13,12 @@ import numpy as np
 from . import _lib
 from ._lib import SmiParams, check
 
-__all__ = ["Simulation", "FIELDS", "PLAIN_FP64", "MIXED", "DEPOSIT_ATOMIC", "DEPOSIT_DETERMINISTIC"]
+__all__ = ["Simulation", "FIELDS", "PLAIN_FP64", "MIXED", "DEPOSIT_ATOMIC", "DEPOSIT_DETERMINISTIC",
+           "TUNE_INTERLEAVE", "TUNE_THREADS", "TUNE_PERSISTENT", "TUNE_CELL_ORDER"]
 
 PLAIN_FP64, MIXED = 0, 1
 DEPOSIT_ATOMIC, DEPOSIT_DETERMINISTIC = 0, 1
+TUNE_INTERLEAVE, TUNE_THREADS, TUNE_PERSISTENT, TUNE_CELL_ORDER = 1, 2, 3, 4
 FIELDS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"]
 _FID = {n: i for i, n in enumerate(FIELDS)}
 _D = ctypes.POINTER(ctypes.c_double)
@@ -87,6 +90,23 @@ class Simulation:
     def init_species(self, s, ppc, density, sigma, drift_ux=0.0, radius=0.0, seed=0):
         check(self._L.smi_init_species(self.h, s, int(ppc), float(density), float(sigma), float(drift_ux),
                                        float(radius), int(seed)))
+
+    # ---- launch tuning
+    def set_tuning(self, key, value):
+        check(self._L.smi_set_tuning(self.h, int(key), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        check(self._L.smi_get_tuning(self.h, int(key), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, reps=2):
+        """Pick the particle-kernel launch shape by timing the candidates on
+        the current state; returns the chosen knobs and their time."""
+        ms = ctypes.c_float()
+        check(self._L.smi_autotune(self.h, int(reps), ctypes.byref(ms)))
+        return dict(interleave=self.get_tuning(TUNE_INTERLEAVE), threads=self.get_tuning(TUNE_THREADS),
+                    persistent=self.get_tuning(TUNE_PERSISTENT), particles_ms=ms.value)
 
     def init_species_profile(self, s, density, ppc, sigma=0.0, drift_ux=0.0, seed=0):
         """Per-cell density and particles-per-cell arrays (shape of the grid,

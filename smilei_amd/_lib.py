@@ -60,6 +60,8 @@ SIGNATURES = {
     "smi_profile_step": (ctypes.c_int, [_VP, _F]),
     "smi_stream": (_VP, [_VP]),
     "smi_set_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "smi_get_tuning": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "smi_autotune": (ctypes.c_int, [_VP, ctypes.c_int, _F]),
     "smi_interpolate_3d": (ctypes.c_int, [ctypes.c_int] + [_VP] * 6 + [_I3, _I3, _D3, ctypes.c_int64] + [_VP] * 3
                            + [_VP] * 4 + [_VP]),
     "smi_interpolate_2d": (ctypes.c_int, [ctypes.c_int] + [_VP] * 6 + [ctypes.c_int * 2, ctypes.c_int * 2,

@@ -84,12 +84,18 @@ struct smi_context {
     DetSpecies *d_det = nullptr;   // device copy of the deposit descriptors
     int d_det_n = 0;
     MixSpecies *d_mix = nullptr;   // mixed-form particle-launch descriptors [2][species]
+    FpSpecies *d_fp = nullptr;     // FP64-form particle-launch descriptors [2][species]
     double *partials = nullptr;    // reduction scratch
     double *rho = nullptr;         // diagnostics scratch (ncells)
     int *err = nullptr;            // device flag raised by kernels on corrupt particles
     int cur = 0;                   // particle buffer holding the current state
     bool mixed = false;            // SMI_MIXED form
     int interleave = 1;            // lane-to-particle mapping of the particle kernel (tuning)
+    int threads = 256;             // workgroup size of the particle kernel (tuning)
+    int persistent = 0;            // persistent work-list form of the mixed kernel (tuning)
+    int cell_order = 1;            // canonical order inside a cell (atomic deposit may drop it)
+    int persistent_blocks = 0;     // its grid, from the occupancy of the current knobs
+    int *d_work = nullptr;         // work list (one int per cluster) + head/count (2 ints)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev[6] = {};
 
@@ -141,6 +147,7 @@ void upload_det(smi_context *c)
             v.push_back(d);
         }
     SMI_HIP(hipMemcpyAsync(c->d_det, v.data(), v.size() * sizeof(DetSpecies), hipMemcpyHostToDevice, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));  // v is released on return
 }
 
 // Mixed-form launch descriptors for both ping-pong states (set k: input
@@ -163,6 +170,64 @@ void upload_mix(smi_context *c)
             v.push_back(m);
         }
     SMI_HIP(hipMemcpyAsync(c->d_mix, v.data(), v.size() * sizeof(MixSpecies), hipMemcpyHostToDevice, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));  // v is released on return
+}
+
+// Grid of the persistent mixed kernel for the current knobs (queried outside
+// any graph capture).
+void refresh_persistent(smi_context *c)
+{
+    c->persistent_blocks = c->mixed && c->persistent ? mixed_persistent_blocks(c->g, c->interleave != 0, c->threads) : 0;
+}
+
+// The particle kernel launches of one step reading buffer a, writing 1 - a.
+void enqueue_particles(smi_context *c, int a)
+{
+    const Geom &g = c->g;
+    const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
+    FieldSet f = c->fields();
+    JPtrs J{f.J[0], f.J[1], f.J[2]};
+    GatherFields GF{f.E[0], f.E[1], f.E[2], f.Bm[0], f.Bm[1], f.Bm[2]};
+    if (c->mixed) {
+        // species share the cluster's field and J tiles, kMaxMerged per launch
+        const int ns = (int)c->sp.size();
+        for (int g0 = 0; g0 < ns; g0 += kMaxMerged) {
+            const int n = std::min(kMaxMerged, ns - g0);
+            double cr_max = 0.;
+            for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
+            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->interleave != 0,
+                                   c->threads, c->persistent ? c->persistent_blocks : 0, c->d_work,
+                                   c->d_work + g.npatches * g.clpp, c->st);
+        }
+    } else {
+        const int ns = (int)c->sp.size();
+        for (int g0 = 0; g0 < ns; g0 += kMaxMerged)
+            launch_particles(g, c->d_fp + a * ns + g0, std::min(kMaxMerged, ns - g0), GF, J, !det, c->err,
+                             c->threads, c->st);
+    }
+}
+
+// FP64-form launch descriptors for both ping-pong states.
+void upload_fp(smi_context *c)
+{
+    if (c->sp.empty() || c->mixed) return;
+    std::vector<FpSpecies> v;
+    for (int k = 0; k < 2; ++k)
+        for (auto &s : c->sp) {
+            FpSpecies f;
+            f.k = s.k;
+            f.A = s.view(k);
+            f.B = s.view(1 - k);
+            f.perm = s.sb.perm;
+            f.cell_off = s.sb.cell_off;
+            f.keys = s.keys[1 - k];
+            f.counts = s.sb.counts;
+            f.scratch = s.scratch;
+            f.cap = s.cap;
+            v.push_back(f);
+        }
+    SMI_HIP(hipMemcpyAsync(c->d_fp, v.data(), v.size() * sizeof(FpSpecies), hipMemcpyHostToDevice, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));  // v is released on return
 }
 
 // One PIC time step reading particle buffer c->cur and writing 1 - c->cur.
@@ -173,24 +238,10 @@ void enqueue_step(smi_context *c, bool timed)
     const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
     FieldSet f = c->fields();
     JPtrs J{f.J[0], f.J[1], f.J[2]};
-    GatherFields GF{f.E[0], f.E[1], f.E[2], f.Bm[0], f.Bm[1], f.Bm[2]};
     if (timed) SMI_HIP(hipEventRecord(c->ev[0], c->st));
     if (!det)
         for (int q = 0; q < 3; ++q) launch_fill(f.J[q], g.ncells, 0.0, c->st);  // restartRhoJ
-    if (c->mixed) {
-        // species share the cluster's field and J tiles, kMaxMerged per launch
-        const int ns = (int)c->sp.size();
-        for (int g0 = 0; g0 < ns; g0 += kMaxMerged) {
-            const int n = std::min(kMaxMerged, ns - g0);
-            double cr_max = 0.;
-            for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
-            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->interleave != 0, c->st);
-        }
-    } else {
-        for (auto &s : c->sp)
-            launch_particles(g, s.k, s.view(a), s.sb.perm, s.view(1 - a), s.keys[1 - a], s.scratch, s.cap,
-                             s.sb.cell_off, GF, J, !det, c->err, s.sb.counts, c->st);
-    }
+    enqueue_particles(c, a);
     if (timed) SMI_HIP(hipEventRecord(c->ev[1], c->st));
     if (det && !c->sp.empty()) {
         launch_det_deposit(g, c->d_det + a * c->d_det_n, (int)c->sp.size(), c->pj, c->err, c->st);
@@ -199,7 +250,8 @@ void enqueue_step(smi_context *c, bool timed)
     if (timed) SMI_HIP(hipEventRecord(c->ev[2], c->st));
     launch_maxwell(g, f, c->st);
     if (timed) SMI_HIP(hipEventRecord(c->ev[3], c->st));
-    for (auto &s : c->sp) launch_sort(g, s.keys[1 - a], s.n, s.sb, c->st, /*counted=*/true);
+    const bool ordered = det || c->cell_order;
+    for (auto &s : c->sp) launch_sort(g, s.keys[1 - a], s.n, s.sb, c->st, /*counted=*/true, ordered);
     if (timed) SMI_HIP(hipEventRecord(c->ev[4], c->st));
 }
 
@@ -354,6 +406,8 @@ int smi_destroy(smi_context *c)
     (void)hipFree(c->pj);
     (void)hipFree(c->d_det);
     (void)hipFree(c->d_mix);
+    (void)hipFree(c->d_fp);
+    (void)hipFree(c->d_work);
     (void)hipFree(c->partials);
     (void)hipFree(c->rho);
     (void)hipFree(c->err);
@@ -409,9 +463,16 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     upload_det(c);
     (void)hipFree(c->d_mix);
     c->d_mix = nullptr;
+    (void)hipFree(c->d_fp);
+    c->d_fp = nullptr;
+    if (!c->mixed) {
+        c->d_fp = dalloc<FpSpecies>(2 * c->sp.size());
+        upload_fp(c);
+    }
     if (c->mixed) {
         c->d_mix = dalloc<MixSpecies>(2 * c->sp.size());
         upload_mix(c);
+        if (!c->d_work) c->d_work = dalloc<int>((int64_t)c->g.npatches * c->g.clpp + 2);
     }
     SMI_HIP(hipStreamSynchronize(c->st));
     if (id) *id = (int)c->sp.size() - 1;
@@ -759,10 +820,89 @@ int smi_set_tuning(smi_context *c, int key, int value)
     check_ctx(c);
     if (key == SMI_TUNE_INTERLEAVE) {
         c->interleave = value != 0;
-        drop_graphs(c);
+    } else if (key == SMI_TUNE_THREADS) {
+        if (value != 64 && value != 128 && value != 256 && value != 512)
+            throw Error(SMI_ERR_INVALID, "SMI_TUNE_THREADS must be 64, 128, 256 or 512");
+        if (value > particle_max_threads(c->g, c->mixed))
+            throw Error(SMI_ERR_INVALID, "SMI_TUNE_THREADS above the particle kernel's limit (256 for the 3D "
+                                         "order-4 FP64 kernel)");
+        c->threads = value;
+    } else if (key == SMI_TUNE_CELL_ORDER) {
+        c->cell_order = value != 0;
+    } else if (key == SMI_TUNE_PERSISTENT) {
+        if (value && !c->mixed) throw Error(SMI_ERR_INVALID, "SMI_TUNE_PERSISTENT applies to the mixed form");
+        c->persistent = value != 0;
     } else {
         throw Error(SMI_ERR_INVALID, "unknown tuning key");
     }
+    drop_graphs(c);
+    refresh_persistent(c);
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_get_tuning(smi_context *c, int key, int *value)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (!value) throw Error(SMI_ERR_INVALID, "null output");
+    if (key == SMI_TUNE_INTERLEAVE) *value = c->interleave;
+    else if (key == SMI_TUNE_THREADS) *value = c->threads;
+    else if (key == SMI_TUNE_PERSISTENT) *value = c->persistent;
+    else if (key == SMI_TUNE_CELL_ORDER) *value = c->cell_order || c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
+    else throw Error(SMI_ERR_INVALID, "unknown tuning key");
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_autotune(smi_context *c, int reps, float *best_ms)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (reps < 1) throw Error(SMI_ERR_INVALID, "reps must be >= 1");
+    SMI_HIP(hipSetDevice(c->p.device));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    drop_graphs(c);
+    // candidates: lane mapping (mixed form) x workgroup size; each timed on
+    // the live particle state with the particle launches of one step.  The
+    // trial launches only write the output ping-pong buffer (rewritten by the
+    // next step), the destination histograms (cleared below) and, in the
+    // atomic deposit, J (cleared at the start of every step).
+    const int maps[2] = {1, 0};
+    const int nmap = c->mixed ? 2 : 1, npers = c->mixed ? 2 : 1;
+    const int sizes[3] = {128, 256, 512};
+    int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent;
+    float best = 3.4e38f;
+    for (int pm = 0; pm < npers; ++pm)
+    for (int m = 0; m < nmap; ++m)
+        for (int z = 0; z < 3; ++z) {
+            if (sizes[z] > particle_max_threads(c->g, c->mixed)) continue;
+            c->interleave = c->mixed ? maps[m] : c->interleave;
+            c->threads = sizes[z];
+            c->persistent = pm;
+            refresh_persistent(c);
+            enqueue_particles(c, c->cur);  // warm-up
+            SMI_HIP(hipEventRecord(c->ev[0], c->st));
+            for (int r = 0; r < reps; ++r) enqueue_particles(c, c->cur);
+            SMI_HIP(hipEventRecord(c->ev[1], c->st));
+            SMI_HIP(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            SMI_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            ms /= reps;
+            if (ms < best) {
+                best = ms;
+                best_map = c->interleave;
+                best_threads = c->threads;
+                best_pers = c->persistent;
+            }
+        }
+    c->interleave = best_map;
+    c->threads = best_threads;
+    c->persistent = best_pers;
+    refresh_persistent(c);
+    for (auto &s : c->sp) launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    if (best_ms) *best_ms = best;
     return SMI_OK;
     SMI_CATCH
 }

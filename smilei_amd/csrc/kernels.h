@@ -36,19 +36,31 @@ struct JPtrs {
 };
 
 // ---- particle step (particles.hip) ------------------------------------
-// One block per 4-cell cluster of every patch.  Reads the sorted particle
-// list through `perm` from A, writes the advanced particles to B in sorted
-// slot order, their packed destination keys (dir<<27 | global cell) and,
-// when `deposit`, the Esirkepov currents into J (LDS tile + FP64 atomics).
-// When !deposit, `scratch` (7*cap doubles) receives the old and the new
-// (unwrapped) positions and 1/gamma for the deterministic deposit.  A
-// particle found outside its cluster's stencil tile (corrupt input) is left
-// unchanged and raises *err.  The kernel also accumulates the histogram of
-// destination cells into `counts` (zero on entry) for launch_sort.
-void launch_particles(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
-                      const PartPtrs &B, uint32_t *keys, double *scratch, int64_t cap,
-                      const int32_t *cell_off, const GatherFields &F, const JPtrs &J, bool deposit,
-                      int *err, int32_t *counts, hipStream_t st);
+constexpr int kMaxMerged = 4;  // species per merged particle launch
+// One species of a particle launch of the FP64 form (device array).
+struct FpSpecies {
+    SpeciesK k;
+    PartPtrs A, B;                   // input / output buffers
+    const int32_t *perm, *cell_off;  // sorted view of the input
+    uint32_t *keys;                  // destination keys written for B
+    int32_t *counts;                 // destination histogram
+    double *scratch;                 // 7*cap doubles (deterministic deposit)
+    int64_t cap;
+};
+// One block per 4-cell cluster of every patch runs the particles of the ns
+// <= kMaxMerged species.  Reads each sorted particle list through `perm`
+// from A, writes the advanced particles to B in sorted slot order, their
+// packed destination keys (dir<<27 | global cell) and, when `deposit`, the
+// Esirkepov currents into J (one LDS tile per cluster + FP64 atomics).
+// When !deposit, `scratch` receives the old and the new (unwrapped)
+// positions and 1/gamma for the deterministic deposit.  A particle found
+// outside its cluster's stencil tile (corrupt input) is left unchanged and
+// raises *err.  The kernel also accumulates the histogram of destination
+// cells into `counts` (zero on entry) for launch_sort.
+void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const GatherFields &F, const JPtrs &J,
+                      bool deposit, int *err, int threads, hipStream_t st);
+// Largest workgroup size the particle kernel of this form accepts.
+int particle_max_threads(const Geom &g, bool mixed);
 
 // Deterministic deposit into per-patch ghosted J arrays (reference layout,
 // particle order of the patch).  `pj` holds npatches * (|Jx|+|Jy|+|Jz|).
@@ -76,11 +88,12 @@ struct SortBufs {
 };
 int64_t sort_bsum_size(int64_t ncells);
 // Counting sort of particles by destination cell, ties by (direction,
-// slot): fills cell_off and perm from keys[0..n).  With counted = true the
-// histogram of destination cells is already in sb.counts (the particle
-// kernel builds it); otherwise it is computed from the keys first.
+// slot) when in_cell_order (otherwise the order inside a cell is that of
+// the atomic scatter): fills cell_off and perm from keys[0..n).  With
+// counted = true the histogram of destination cells is already in sb.counts
+// (the particle kernel builds it); otherwise it is computed from the keys.
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st,
-                 bool counted = false);
+                 bool counted = false, bool in_cell_order = true);
 // Keys of freshly loaded particles (positions in A, slot order = input).
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
 
@@ -88,7 +101,6 @@ void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *k
 // Same contract as launch_particles (always the atomic tile deposit); the
 // cell of particle i is keysA[i], its new key goes to keysB[slot].
 // One species of a merged particle launch of the mixed form (device array).
-constexpr int kMaxMerged = 4;
 struct MixSpecies {
     SpeciesKM k;
     MixPtrs A, B;                   // input / output buffers
@@ -99,9 +111,14 @@ struct MixSpecies {
 };
 // Particle kernel of the mixed form over ns <= kMaxMerged species that share
 // one J tile per cluster; cr_max = the largest current contribution of those
-// species (sets the fixed-point unit of the tile).
+// species (sets the fixed-point unit of the tile).  persistent_blocks > 0
+// selects the persistent form: that many workgroups pull the non-empty
+// clusters from a work list (work: one int per cluster, wcount: 2 ints).
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, bool interleave, hipStream_t st);
+                            const JPtrs &J, int *err, bool interleave, int threads, int persistent_blocks,
+                            int *work, int *wcount, hipStream_t st);
+// Resident workgroups of the persistent mixed kernel on the current device.
+int mixed_persistent_blocks(const Geom &g, bool interleave, int threads);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
 void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm, int64_t n,
                        const PartPtrs &X, hipStream_t st);

@@ -57,12 +57,12 @@ __device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, c
 // ---------------------------------------------------------------------------
 // Fused particle kernel.
 // ---------------------------------------------------------------------------
+// One workgroup per cluster runs the particles of every species of the
+// launch (specs[0..ns)) against one field tile (and, with DEPOSIT, one J
+// tile).
 template <int ND, int ORDER, bool DEPOSIT>
-__global__ void __launch_bounds__(256, 3)
-k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, PartPtrs B,
-            uint32_t *__restrict__ keys, double *__restrict__ scratch, int64_t cap,
-            const int32_t *__restrict__ cell_off, GatherFields F, JPtrs J, int *__restrict__ err,
-            int32_t *__restrict__ counts)
+__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
+k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err)
 {
     using TL = Tile<ORDER>;
     constexpr int H = TL::H, GT = TL::GT, JT = TL::JT, S = TL::S;
@@ -70,7 +70,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     constexpr int JT3 = ND == 3 ? JT * JT * JT : JT * JT;
     __shared__ double sF[6 * GT3];
     __shared__ double sJ[DEPOSIT ? 3 * JT3 : 1];
-    __shared__ int sHist[64];  // destination-cell counts of this cluster's own cells
+    __shared__ int sHist[kMaxMerged][64];  // destination-cell counts of this cluster's own cells
     // 3D deposit: particles whose node changes on some axis are deposited in
     // a second pass over this queue (old delta, new position in cells, charge
     // weight, old node relative to the cluster), so that the main loop runs
@@ -81,8 +81,10 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     __shared__ int sQn;
 
     const Cluster cl = decode_cluster(g, blockIdx.x);
-    const int s = cell_off[cl.gbase], e = cell_off[cl.gbase + g.cells_per_cl];
-    if (s == e) return;  // uniform across the workgroup
+    int total = 0;
+    for (int q = 0; q < ns; ++q)
+        total += specs[q].cell_off[cl.gbase + g.cells_per_cl] - specs[q].cell_off[cl.gbase];
+    if (total == 0) return;  // uniform across the workgroup
 
     // ---- stage E, B_m of the cluster + halo
     const double *src[6] = {F.Ex, F.Ey, F.Ez, F.Bx, F.By, F.Bz};
@@ -106,7 +108,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     }
     if constexpr (DEPOSIT)
         for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.;
-    if (threadIdx.x < 64) sHist[threadIdx.x] = 0;
+    for (int t = threadIdx.x; t < kMaxMerged * 64; t += blockDim.x) sHist[t / 64][t % 64] = 0;
     if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
 
@@ -120,6 +122,16 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
     // Particles of the cluster are sorted by cell (about ppc per cell).  Lane l
     // of every wavefront takes slot l*q + row, so the 64 lanes of a wave work
     // in different cells and their LDS atomics rarely hit the same address.
+    for (int sid = 0; sid < ns; ++sid) {
+    const FpSpecies &Sp = specs[sid];
+    const SpeciesK &sp = Sp.k;
+    const PartPtrs A = Sp.A, B = Sp.B;
+    const int32_t *__restrict__ perm = Sp.perm;
+    uint32_t *__restrict__ keys = Sp.keys;
+    int32_t *__restrict__ counts = Sp.counts;
+    double *__restrict__ scratch = Sp.scratch;
+    const int64_t cap = Sp.cap;
+    const int s = Sp.cell_off[cl.gbase], e = Sp.cell_off[cl.gbase + g.cells_per_cl];
     const int N = e - s;
     const int q = DEPOSIT ? (N + 63) >> 6 : 1;
     const int span = DEPOSIT ? 64 * q : N;
@@ -156,7 +168,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
             // not in this cluster (corrupt state): leave it untouched, flag it
             atomicOr(err, 1);
             keys[jslot] = (13u << 27) | (uint32_t)(cl.gbase);
-            atomicAdd(&sHist[0], 1);
+            atomicAdd(&sHist[sid][0], 1);
             B.x[jslot] = x[0];
             B.y[jslot] = x[1];
             if constexpr (ND == 3) B.z[jslot] = x[2];
@@ -215,7 +227,7 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
         // histogram of destination cells for the sort: own-cluster cells in
         // LDS, particles leaving the cluster straight to the global counts
         const int64_t rel_cell = (int64_t)gkey - cl.gbase;
-        if (rel_cell >= 0 && rel_cell < g.cells_per_cl) atomicAdd(&sHist[rel_cell], 1);
+        if (rel_cell >= 0 && rel_cell < g.cells_per_cl) atomicAdd(&sHist[sid][rel_cell], 1);
         else atomicAdd(&counts[gkey], 1);
         B.x[jslot] = xw[0];
         B.y[jslot] = xw[1];
@@ -283,10 +295,12 @@ k_particles(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm, P
         }
     }
 
+    }  // species
+
     __syncthreads();
-    if (threadIdx.x < g.cells_per_cl) {
-        const int c = sHist[threadIdx.x];
-        if (c) atomicAdd(&counts[cl.gbase + threadIdx.x], c);
+    for (int t = threadIdx.x; t < ns * 64; t += blockDim.x) {
+        const int c = t % 64 < g.cells_per_cl ? sHist[t / 64][t % 64] : 0;
+        if (c) atomicAdd(&specs[t / 64].counts[cl.gbase + t % 64], c);
     }
     if constexpr (QUEUE) {
         const int nq = sQn < QCAP ? sQn : QCAP;
@@ -539,15 +553,17 @@ __global__ void k_fold_interior(Geom g, const double *__restrict__ pj, JPtrs J)
 
 int64_t patch_j_size(const Geom &g) { return pj_layout(g).per; }
 
-void launch_particles(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
-                      const PartPtrs &B, uint32_t *keys, double *scratch, int64_t cap,
-                      const int32_t *cell_off, const GatherFields &F, const JPtrs &J, bool deposit,
-                      int *err, int32_t *counts, hipStream_t st)
+void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const GatherFields &F, const JPtrs &J,
+                      bool deposit, int *err, int threads, hipStream_t st)
 {
+    if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+        throw Error(-1, "particle-kernel workgroup size must be 64, 128, 256 or 512");
+    if (threads > particle_max_threads(g, false))
+        throw Error(-1, "workgroup size above the particle kernel's limit for this geometry and order");
+    if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
     const int nb = g.npatches * g.clpp;
-    const dim3 grid(nb), block(256);
-#define SMI_PK(ND, O, D) hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, sp, A, perm, B, keys, \
-                                            scratch, cap, cell_off, F, J, err, counts)
+    const dim3 grid(nb), block(threads);
+#define SMI_PK(ND, O, D) hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, specs, ns, F, J, err)
     if (g.nd == 3 && g.order == 2) {
         if (deposit) SMI_PK(3, 2, true);
         else SMI_PK(3, 2, false);
@@ -562,6 +578,12 @@ void launch_particles(const Geom &g, const SpeciesK &sp, const PartPtrs &A, cons
     }
 #undef SMI_PK
     SMI_LAUNCH_CHECK();
+}
+
+int particle_max_threads(const Geom &g, bool mixed)
+{
+    // the 3D order-4 FP64 kernel needs the full register file of one wave
+    return !mixed && g.nd == 3 && g.order == 4 ? 256 : 512;
 }
 
 void launch_det_deposit(const Geom &g, const DetSpecies *species, int nspecies, double *pj, int *err,

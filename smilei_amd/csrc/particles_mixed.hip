@@ -17,6 +17,8 @@
 // Same call chain as the FP64 kernel: Interpolator3D{2,4}Order (gather),
 // PusherBoris, PartBoundCond (direction of motion), Cluster3D::CellKey,
 // Projector3D{2,4}Order::currents (Esirkepov).
+#include <algorithm>
+
 #include "kernels.h"
 #include "pic_device.h"
 
@@ -251,12 +253,13 @@ __device__ inline uint32_t global_key3(const Geom &g, const int *c, int *pc)
     return (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
 }
 
-// One workgroup per 4^3 cluster runs the particles of every species of the
-// launch (specs[0..ns)) against one field tile and one J tile.
+// One workgroup runs the particles of every species of the launch
+// (specs[0..ns)) of the 4^3 cluster `cluster` against one field tile and
+// one J tile.  Uniform across the workgroup; returns at once for an empty
+// cluster.
 template <int ORDER, bool INTERLEAVE>
-__global__ void __launch_bounds__(256, ORDER == 2 ? 3 : 2)
-k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
-                  int *__restrict__ err)
+__device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict__ specs, int ns, double cr_max,
+                                     const GatherFields &F, const JPtrs &J, int *__restrict__ err, int cluster)
 {
     constexpr int H = ORDER / 2;
     // J tile: stencil rows [c0-H-1, c0+4+H]
@@ -274,7 +277,7 @@ k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double c
     __shared__ float sQ[QCAP][8];
     __shared__ int sQn;
 
-    const Cl cl = cluster_of(g, blockIdx.x);
+    const Cl cl = cluster_of(g, cluster);
     int total = 0;
     for (int q = 0; q < ns; ++q) total += specs[q].cell_off[cl.gbase + 64] - specs[q].cell_off[cl.gbase];
     if (total == 0) return;
@@ -490,6 +493,56 @@ k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double c
     }
 }
 
+// One workgroup per cluster of the grid.
+template <int ORDER, bool INTERLEAVE>
+__global__ void __launch_bounds__(512, ORDER == 2 ? 3 : 2)
+k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
+                  int *__restrict__ err)
+{
+    mixed_cluster<ORDER, INTERLEAVE>(g, specs, ns, cr_max, F, J, err, blockIdx.x);
+}
+
+// Persistent form: a grid sized to the resident workgroups pulls clusters
+// from the work list (the non-empty clusters, wcount[0] of them) through the
+// queue head wcount[1]; every workgroup leaves once the list is exhausted.
+template <int ORDER, bool INTERLEAVE>
+__global__ void __launch_bounds__(512, ORDER == 2 ? 3 : 2)
+k_particles_mixed_list(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
+                       int *__restrict__ err, const int *__restrict__ work, int *__restrict__ wcount)
+{
+    __shared__ int sItem;
+    const int nwork = wcount[0];
+    for (;;) {
+        __syncthreads();  // the previous cluster is done with the LDS tiles
+        if (threadIdx.x == 0) sItem = atomicAdd(&wcount[1], 1);
+        __syncthreads();
+        const int item = sItem;
+        if (item >= nwork) break;
+        mixed_cluster<ORDER, INTERLEAVE>(g, specs, ns, cr_max, F, J, err, work[item]);
+    }
+}
+
+// Work list of the clusters holding particles of any species of the launch
+// (order of the list is irrelevant).  wcount[0..1] must be zero.
+__global__ void k_worklist(Geom g, const MixSpecies *__restrict__ specs, int ns, int *__restrict__ work,
+                           int *__restrict__ wcount)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = 0;
+    if (c < g.npatches * g.clpp) {
+        const Cl cl = cluster_of(g, c);
+        for (int q = 0; q < ns; ++q) total += specs[q].cell_off[cl.gbase + 64] - specs[q].cell_off[cl.gbase];
+    }
+    const uint64_t m = __ballot(total > 0);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&wcount[0], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (total > 0) work[base + __popcll(m & ((1ull << lane) - 1))] = c;
+}
+
 // FP64 global positions -> (cell key, FP32 offsets); FP32 momenta/weight.
 __global__ void k_to_mixed(Geom g, PartPtrs X, int64_t n, MixPtrs M, uint32_t *__restrict__ keys)
 {
@@ -607,21 +660,59 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }  // namespace
 
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, bool interleave, hipStream_t st)
+                            const JPtrs &J, int *err, bool interleave, int threads, int persistent_blocks,
+                            int *work, int *wcount, hipStream_t st)
 {
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
-    const dim3 grid(g.npatches * g.clpp), block(256);
-#define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), grid, block, 0, st, g, specs, ns, cr_max, F, J, err)
-    if (g.order == 2) {
-        if (interleave) SMI_PM(2, true);
-        else SMI_PM(2, false);
-    } else {
-        if (interleave) SMI_PM(4, true);
-        else SMI_PM(4, false);
-    }
+    if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+        throw Error(-1, "particle-kernel workgroup size must be 64, 128, 256 or 512");
+    const int nclusters = g.npatches * g.clpp;
+    const dim3 block(threads);
+    if (persistent_blocks <= 0) {
+#define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), dim3(nclusters), block, 0, st, g, specs, ns, \
+                                        cr_max, F, J, err)
+        if (g.order == 2) {
+            if (interleave) SMI_PM(2, true);
+            else SMI_PM(2, false);
+        } else {
+            if (interleave) SMI_PM(4, true);
+            else SMI_PM(4, false);
+        }
 #undef SMI_PM
+        SMI_LAUNCH_CHECK();
+        return;
+    }
+    // persistent: build the work list, then as many workgroups as fit at once
+    launch_fill_i32(wcount, 2, 0, st);
+    hipLaunchKernelGGL(k_worklist, dim3((nclusters + 255) / 256), dim3(256), 0, st, g, specs, ns, work, wcount);
     SMI_LAUNCH_CHECK();
+    const int nblocks = std::min(nclusters, persistent_blocks);
+#define SMI_PL(O, I) hipLaunchKernelGGL((k_particles_mixed_list<O, I>), dim3(nblocks), block, 0, st, g, specs, ns, \
+                                        cr_max, F, J, err, (const int *)work, wcount)
+    if (g.order == 2) {
+        if (interleave) SMI_PL(2, true);
+        else SMI_PL(2, false);
+    } else {
+        if (interleave) SMI_PL(4, true);
+        else SMI_PL(4, false);
+    }
+#undef SMI_PL
+    SMI_LAUNCH_CHECK();
+}
+
+int mixed_persistent_blocks(const Geom &g, bool interleave, int threads)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    SMI_HIP(hipGetDevice(&dev));
+    SMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const void *fn = nullptr;
+    if (g.order == 2)
+        fn = interleave ? (const void *)k_particles_mixed_list<2, true> : (const void *)k_particles_mixed_list<2, false>;
+    else
+        fn = interleave ? (const void *)k_particles_mixed_list<4, true> : (const void *)k_particles_mixed_list<4, false>;
+    SMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
+    return std::max(1, cus * std::max(per_cu, 1));
 }
 
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st)

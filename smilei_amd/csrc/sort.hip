@@ -235,7 +235,8 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 int64_t sort_bsum_size(int64_t ncells) { return (ncells + kScanTile - 1) / kScanTile + 1; }
 
-void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st, bool counted)
+void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st, bool counted,
+                 bool in_cell_order)
 {
     const int64_t nc = g.ncells;
     if (n > 0 && !counted) {
@@ -255,9 +256,11 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
         hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
                            sb.counts, sb.perm);
         SMI_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                           (const int32_t *)sb.cell_off, sb.perm);
-        SMI_LAUNCH_CHECK();
+        if (in_cell_order) {
+            hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
+                               (const int32_t *)sb.cell_off, sb.perm);
+            SMI_LAUNCH_CHECK();
+        }
     }
 }
 

@@ -1,0 +1,119 @@
+"""Launch-shape knobs and the autotuner: they change neither the physics
+nor the simulation state."""
+import numpy as np
+import pytest
+
+import smilei_amd as sa
+from tests.plasma import cfl_dt, load, thermal
+
+pytestmark = pytest.mark.gpu
+
+SPECIES = [("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))]
+
+
+def make(precision, deposit=sa.DEPOSIT_ATOMIC, shape=(32, 32, 32), seed=9):
+    dx = (0.2, 0.2, 0.2)
+    sim = sa.Simulation("3Dcartesian", 2, shape, (16, 16, 16), dx, cfl_dt(dx), precision=precision, deposit=deposit)
+    for k, (name, m, q, sig) in enumerate(SPECIES):
+        p = thermal(shape, dx, 8, sig, seed + k)
+        load(sim, sim.add_species(name, m, q, len(p["x"])), p)
+    return sim
+
+
+@pytest.mark.parametrize("threads", [64, 128, 512])
+def test_deterministic_form_independent_of_workgroup_size(threads):
+    a = make(sa.PLAIN_FP64, sa.DEPOSIT_DETERMINISTIC)
+    b = make(sa.PLAIN_FP64, sa.DEPOSIT_DETERMINISTIC)
+    b.set_tuning(sa.TUNE_THREADS, threads)
+    a.step(5)
+    b.step(5)
+    for f in ("Ex", "Jy", "Bz_m"):
+        np.testing.assert_array_equal(a.get_field(f), b.get_field(f))
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["x"], pb["x"])
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+
+
+@pytest.mark.parametrize("precision", [sa.MIXED, sa.PLAIN_FP64])
+def test_autotune_keeps_state(precision):
+    a = make(precision)
+    b = make(precision)
+    a.step(2)
+    b.step(2)
+    knobs = b.autotune(reps=2)
+    assert knobs["threads"] in (128, 256, 512) and knobs["interleave"] in (0, 1)
+    assert knobs["persistent"] in ((0, 1) if precision == sa.MIXED else (0,))
+    assert knobs["particles_ms"] > 0
+    a.step(5)
+    b.step(5)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+        np.testing.assert_allclose(pb["x"], pa["x"], rtol=1e-10)
+    for f in ("Ex", "Jx", "By"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
+
+
+def test_tuning_rejects_bad_values():
+    sim = make(sa.MIXED, shape=(16, 16, 16))
+    with pytest.raises(RuntimeError, match="SMI_TUNE_THREADS"):
+        sim.set_tuning(sa.TUNE_THREADS, 96)
+    with pytest.raises(RuntimeError, match="unknown tuning key"):
+        sim.set_tuning(7, 1)
+    fp64 = make(sa.PLAIN_FP64, shape=(16, 16, 16))
+    with pytest.raises(RuntimeError, match="mixed form"):
+        fp64.set_tuning(sa.TUNE_PERSISTENT, 1)
+
+
+@pytest.mark.parametrize("uneven", [False, True])
+def test_persistent_work_list_matches_grid_launch(uneven):
+    """The persistent work-list launch visits exactly the non-empty clusters:
+    same keys and counts, fields equal up to the order of the J flush."""
+    from tests.plasma import lwfa
+    if uneven:
+        shape, patch, dx = (256, 32, 32), (32, 16, 16), (0.1, 0.5, 0.5)
+        a = lwfa(sa.Simulation, sa.MIXED, shape, patch, dx)
+        b = lwfa(sa.Simulation, sa.MIXED, shape, patch, dx)
+        ns = 1
+    else:
+        a, b, ns = make(sa.MIXED), make(sa.MIXED), 2
+    b.set_tuning(sa.TUNE_PERSISTENT, 1)
+    assert b.get_tuning(sa.TUNE_PERSISTENT) == 1
+    a.step(6)
+    b.step(6)
+    for s in range(ns):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+        np.testing.assert_allclose(pb["x"], pa["x"], rtol=1e-10)
+    for f in ("Ey", "Jx", "Bz"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
+
+
+def test_unordered_cells_hold_the_oracle_multisets():
+    """SMI_TUNE_CELL_ORDER = 0 (in-cell order left to the atomic scatter, as
+    the reference's GPU sort): per cell the same particles as the oracle."""
+    from oracle import pyoracle as O
+    shape, dx = (32, 32, 32), (0.2, 0.2, 0.2)
+    g = make(sa.PLAIN_FP64, shape=shape)
+    g.set_tuning(sa.TUNE_CELL_ORDER, 0)
+    assert g.get_tuning(sa.TUNE_CELL_ORDER) == 0
+    o = O.OracleSim(3, 2, shape, (16, 16, 16), dx, cfl_dt(dx))
+    for k, (name, m, q, sig) in enumerate(SPECIES):
+        p = thermal(shape, dx, 8, sig, 9 + k)
+        o.set_particles(o.add_species(m, q), p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    g.step(6)
+    o.step(6)
+    for s in range(2):
+        a, b = g.get_particles(s), o.get_particles(s)
+        np.testing.assert_array_equal(g.patch_counts(s), o.patch_counts(s))
+        ia = np.lexsort((a["z"], a["y"], a["x"], a["key"], a["patch"]))
+        ib = np.lexsort((b["z"], b["y"], b["x"], b["key"], b["patch"]))
+        np.testing.assert_array_equal(a["key"][ia], b["key"][ib])
+        for k in ("x", "y", "z", "px", "py", "pz", "w"):
+            scale = max(np.abs(b[k]).max(), 1e-300)
+            assert np.abs(a[k][ia] - b[k][ib]).max() <= 1e-12 * scale, k
