@@ -15,7 +15,9 @@ The particles are initialised in HBM; one "step" is the full PIC step
 (gather, Boris push, Esirkepov deposit, cell sort + patch exchange, Yee
 update) over every species.  After the warm-up steps the library's
 autotuner picks the particle kernels' launch shape (lane mapping, workgroup
-size) on the live state, outside the timed region.  Both precision forms run in turn (mixed first:
+size, persistent work list) on the live state, outside the timed region.
+The order of particles inside a cell is left to the atomic scatter, as in
+the reference's GPU sort (`--cell-order canonical` adds the in-cell sort).  Both precision forms run in turn (mixed first:
 it is the headline); each is timed over the same number of steps.
 
 Prints one JSON line.  `value` = particles * steps / wall time of the timed
@@ -46,7 +48,7 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
-TRAFFIC_PER_PUSH = {"beam": {"mixed": 272.6, "fp64": 353.9}}
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 189.0, "fp64": 392.2}}
 
 
 def cfl_dt(dx, factor=0.95):
@@ -86,6 +88,7 @@ def run_gpu(args, W, form):
     prec = sa.MIXED if form == "mixed" else sa.PLAIN_FP64
     sim = sa.Simulation("3Dcartesian", 2, W["shape"], W["patch"], W["dx"], W["dt"],
                         precision=prec, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
+    sim.set_tuning(sa.TUNE_CELL_ORDER, 1 if args.cell_order == "canonical" else 0)
     ncell = int(np.prod(W["shape"]))
     per_species = []
     for k, (name, m, q, sig, ppc, dens, drift, radius) in enumerate(W["species"]):
@@ -184,6 +187,9 @@ def main():
     ap.add_argument("--forms", default="mixed,fp64", help="comma list of mixed,fp64; the first is the headline")
     ap.add_argument("--copy-only", action="store_true", help="run the device copy probe only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--cell-order", default="any", choices=["any", "canonical"],
+                    help="order of particles inside a cell after the sort: 'any' (left to the atomic scatter, "
+                         "as the reference's GPU sort) or the canonical order")
     args = ap.parse_args()
     if args.copy_only:
         print(json.dumps({"copy_GBs": copy_probe(), "bytes_per_copy": 4 << 30}))
@@ -227,7 +233,8 @@ def main():
         "dtype": "f32" if head == "mixed" else "f64",
         "data": "synthetic (seeded plasma initialised in HBM)",
         "config": {
-            "workload": "%s (%d particles), order 2, Yee, periodic, atomic tile deposit" % (W["desc"], h["npart"]),
+            "workload": "%s (%d particles), order 2, Yee, periodic, atomic tile deposit, in-cell order %s"
+                        % (W["desc"], h["npart"], args.cell_order),
             "form": head,
             "particles": h["npart"],
             "forms": {f: {k: v for k, v in r.items() if k != "energies"} for f, r in res.items()},
