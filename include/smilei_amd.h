@@ -157,6 +157,10 @@ void *smi_stream(smi_context *ctx);
 #define SMI_TUNE_INTERLEAVE 1
 #define SMI_TUNE_THREADS 2
 #define SMI_TUNE_PERSISTENT 3
+/* SMI_TUNE_XCD: 1 = the grid launch hands each of the chip's 8 XCDs (own L2
+ * each; workgroups are dispatched to them round-robin) a contiguous run of
+ * clusters, so that neighbouring clusters share L2 lines; 0 = grid order. */
+#define SMI_TUNE_XCD 5
 /* Not searched by the autotuner (it changes what smi_get_particles returns):
  * SMI_TUNE_CELL_ORDER 1 (default) keeps the canonical order of particles
  * inside a cell (direction of motion, then previous slot); 0 leaves it to
@@ -169,7 +173,8 @@ int smi_get_tuning(smi_context *ctx, int key, int *value);
 /* Project autotuner (no counterpart in the reference, whose GPU launch shapes
  * are fixed): times the particle-kernel launches of one step on the live
  * particle state for every lane mapping x launch form (mixed form) x
- * workgroup size 128/256/512, `reps` launches each, and keeps the fastest; best_ms (may be
+ * cluster order (SMI_TUNE_XCD) x workgroup size 128/256/512, `reps`
+ * launches each, and keeps the fastest; best_ms (may be
  * NULL) receives its time.  Leaves the simulation state as it was. */
 int smi_autotune(smi_context *ctx, int reps, float *best_ms);
 

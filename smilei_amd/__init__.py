@@ -14,11 +14,11 @@ from . import _lib
 from ._lib import SmiParams, check
 
 __all__ = ["Simulation", "FIELDS", "PLAIN_FP64", "MIXED", "DEPOSIT_ATOMIC", "DEPOSIT_DETERMINISTIC",
-           "TUNE_INTERLEAVE", "TUNE_THREADS", "TUNE_PERSISTENT", "TUNE_CELL_ORDER"]
+           "TUNE_INTERLEAVE", "TUNE_THREADS", "TUNE_PERSISTENT", "TUNE_CELL_ORDER", "TUNE_XCD"]
 
 PLAIN_FP64, MIXED = 0, 1
 DEPOSIT_ATOMIC, DEPOSIT_DETERMINISTIC = 0, 1
-TUNE_INTERLEAVE, TUNE_THREADS, TUNE_PERSISTENT, TUNE_CELL_ORDER = 1, 2, 3, 4
+TUNE_INTERLEAVE, TUNE_THREADS, TUNE_PERSISTENT, TUNE_CELL_ORDER, TUNE_XCD = 1, 2, 3, 4, 5
 FIELDS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"]
 _FID = {n: i for i, n in enumerate(FIELDS)}
 _D = ctypes.POINTER(ctypes.c_double)
@@ -106,7 +106,8 @@ class Simulation:
         ms = ctypes.c_float()
         check(self._L.smi_autotune(self.h, int(reps), ctypes.byref(ms)))
         return dict(interleave=self.get_tuning(TUNE_INTERLEAVE), threads=self.get_tuning(TUNE_THREADS),
-                    persistent=self.get_tuning(TUNE_PERSISTENT), particles_ms=ms.value)
+                    persistent=self.get_tuning(TUNE_PERSISTENT), xcd=self.get_tuning(TUNE_XCD),
+                    particles_ms=ms.value)
 
     def init_species_profile(self, s, density, ppc, sigma=0.0, drift_ux=0.0, seed=0):
         """Per-cell density and particles-per-cell arrays (shape of the grid,

@@ -94,6 +94,7 @@ struct smi_context {
     int threads = 256;             // workgroup size of the particle kernel (tuning)
     int persistent = 0;            // persistent work-list form of the mixed kernel (tuning)
     int cell_order = 1;            // canonical order inside a cell (atomic deposit may drop it)
+    int xcd = 0;                   // XCD-aware cluster order of the grid launch (tuning)
     int persistent_blocks = 0;     // its grid, from the occupancy of the current knobs
     int *d_work = nullptr;         // work list (one int per cluster) + head/count (2 ints)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -196,14 +197,14 @@ void enqueue_particles(smi_context *c, int a)
             double cr_max = 0.;
             for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
             launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->interleave != 0,
-                                   c->threads, c->persistent ? c->persistent_blocks : 0, c->d_work,
+                                   c->threads, c->xcd != 0, c->persistent ? c->persistent_blocks : 0, c->d_work,
                                    c->d_work + g.npatches * g.clpp, c->st);
         }
     } else {
         const int ns = (int)c->sp.size();
         for (int g0 = 0; g0 < ns; g0 += kMaxMerged)
             launch_particles(g, c->d_fp + a * ns + g0, std::min(kMaxMerged, ns - g0), GF, J, !det, c->err,
-                             c->threads, c->st);
+                             c->threads, c->xcd != 0, c->st);
     }
 }
 
@@ -829,6 +830,8 @@ int smi_set_tuning(smi_context *c, int key, int value)
         c->threads = value;
     } else if (key == SMI_TUNE_CELL_ORDER) {
         c->cell_order = value != 0;
+    } else if (key == SMI_TUNE_XCD) {
+        c->xcd = value != 0;
     } else if (key == SMI_TUNE_PERSISTENT) {
         if (value && !c->mixed) throw Error(SMI_ERR_INVALID, "SMI_TUNE_PERSISTENT applies to the mixed form");
         c->persistent = value != 0;
@@ -849,6 +852,7 @@ int smi_get_tuning(smi_context *c, int key, int *value)
     if (key == SMI_TUNE_INTERLEAVE) *value = c->interleave;
     else if (key == SMI_TUNE_THREADS) *value = c->threads;
     else if (key == SMI_TUNE_PERSISTENT) *value = c->persistent;
+    else if (key == SMI_TUNE_XCD) *value = c->xcd;
     else if (key == SMI_TUNE_CELL_ORDER) *value = c->cell_order || c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
     else throw Error(SMI_ERR_INVALID, "unknown tuning key");
     return SMI_OK;
@@ -871,15 +875,18 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     const int maps[2] = {1, 0};
     const int nmap = c->mixed ? 2 : 1, npers = c->mixed ? 2 : 1;
     const int sizes[3] = {128, 256, 512};
-    int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent;
+    int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent, best_xcd = c->xcd;
     float best = 3.4e38f;
     for (int pm = 0; pm < npers; ++pm)
+    for (int xo = 0; xo < 2; ++xo)
     for (int m = 0; m < nmap; ++m)
         for (int z = 0; z < 3; ++z) {
             if (sizes[z] > particle_max_threads(c->g, c->mixed)) continue;
+            if (pm && xo) continue;  // the persistent form takes clusters from its work list
             c->interleave = c->mixed ? maps[m] : c->interleave;
             c->threads = sizes[z];
             c->persistent = pm;
+            c->xcd = xo;
             refresh_persistent(c);
             enqueue_particles(c, c->cur);  // warm-up
             SMI_HIP(hipEventRecord(c->ev[0], c->st));
@@ -894,11 +901,13 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
                 best_map = c->interleave;
                 best_threads = c->threads;
                 best_pers = c->persistent;
+                best_xcd = c->xcd;
             }
         }
     c->interleave = best_map;
     c->threads = best_threads;
     c->persistent = best_pers;
+    c->xcd = best_xcd;
     refresh_persistent(c);
     for (auto &s : c->sp) launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
     SMI_HIP(hipStreamSynchronize(c->st));

@@ -58,7 +58,7 @@ struct FpSpecies {
 // raises *err.  The kernel also accumulates the histogram of destination
 // cells into `counts` (zero on entry) for launch_sort.
 void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const GatherFields &F, const JPtrs &J,
-                      bool deposit, int *err, int threads, hipStream_t st);
+                      bool deposit, int *err, int threads, bool xcd, hipStream_t st);
 // Largest workgroup size the particle kernel of this form accepts.
 int particle_max_threads(const Geom &g, bool mixed);
 
@@ -115,8 +115,8 @@ struct MixSpecies {
 // selects the persistent form: that many workgroups pull the non-empty
 // clusters from a work list (work: one int per cluster, wcount: 2 ints).
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, bool interleave, int threads, int persistent_blocks,
-                            int *work, int *wcount, hipStream_t st);
+                            const JPtrs &J, int *err, bool interleave, int threads, bool xcd,
+                            int persistent_blocks, int *work, int *wcount, hipStream_t st);
 // Resident workgroups of the persistent mixed kernel on the current device.
 int mixed_persistent_blocks(const Geom &g, bool interleave, int threads);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);

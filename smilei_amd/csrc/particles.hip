@@ -62,7 +62,8 @@ __device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, c
 // tile).
 template <int ND, int ORDER, bool DEPOSIT>
 __global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
-k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err)
+k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err,
+            int xcd)
 {
     using TL = Tile<ORDER>;
     constexpr int H = TL::H, GT = TL::GT, JT = TL::JT, S = TL::S;
@@ -80,7 +81,7 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
     __shared__ double sQ[QCAP][8];
     __shared__ int sQn;
 
-    const Cluster cl = decode_cluster(g, blockIdx.x);
+    const Cluster cl = decode_cluster(g, xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x);
     int total = 0;
     for (int q = 0; q < ns; ++q)
         total += specs[q].cell_off[cl.gbase + g.cells_per_cl] - specs[q].cell_off[cl.gbase];
@@ -554,7 +555,7 @@ __global__ void k_fold_interior(Geom g, const double *__restrict__ pj, JPtrs J)
 int64_t patch_j_size(const Geom &g) { return pj_layout(g).per; }
 
 void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const GatherFields &F, const JPtrs &J,
-                      bool deposit, int *err, int threads, hipStream_t st)
+                      bool deposit, int *err, int threads, bool xcd, hipStream_t st)
 {
     if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
         throw Error(-1, "particle-kernel workgroup size must be 64, 128, 256 or 512");
@@ -563,7 +564,8 @@ void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const Gathe
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
     const int nb = g.npatches * g.clpp;
     const dim3 grid(nb), block(threads);
-#define SMI_PK(ND, O, D) hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, specs, ns, F, J, err)
+#define SMI_PK(ND, O, D) hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, specs, ns, F, J, err, \
+                                            xcd ? 1 : 0)
     if (g.nd == 3 && g.order == 2) {
         if (deposit) SMI_PK(3, 2, true);
         else SMI_PK(3, 2, false);

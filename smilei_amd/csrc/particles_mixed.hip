@@ -493,13 +493,14 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     }
 }
 
-// One workgroup per cluster of the grid.
+// One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
 template <int ORDER, bool INTERLEAVE>
 __global__ void __launch_bounds__(512, ORDER == 2 ? 3 : 2)
 k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
-                  int *__restrict__ err)
+                  int *__restrict__ err, int xcd)
 {
-    mixed_cluster<ORDER, INTERLEAVE>(g, specs, ns, cr_max, F, J, err, blockIdx.x);
+    const int b = xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x;
+    mixed_cluster<ORDER, INTERLEAVE>(g, specs, ns, cr_max, F, J, err, b);
 }
 
 // Persistent form: a grid sized to the resident workgroups pulls clusters
@@ -660,8 +661,8 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }  // namespace
 
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, bool interleave, int threads, int persistent_blocks,
-                            int *work, int *wcount, hipStream_t st)
+                            const JPtrs &J, int *err, bool interleave, int threads, bool xcd,
+                            int persistent_blocks, int *work, int *wcount, hipStream_t st)
 {
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
@@ -671,7 +672,7 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
     const dim3 block(threads);
     if (persistent_blocks <= 0) {
 #define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), dim3(nclusters), block, 0, st, g, specs, ns, \
-                                        cr_max, F, J, err)
+                                        cr_max, F, J, err, xcd ? 1 : 0)
         if (g.order == 2) {
             if (interleave) SMI_PM(2, true);
             else SMI_PM(2, false);

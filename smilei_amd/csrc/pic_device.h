@@ -23,6 +23,20 @@ template <int ORDER> struct Tile {
     static constexpr int S = 2 * H + 3;      // Esirkepov stencil length
 };
 
+// XCD-aware order of the one-workgroup-per-cluster grid.  Workgroups are
+// dispatched round-robin over the 8 XCDs of the chip (workgroup b to XCD
+// b % 8), each XCD with its own L2; this mapping gives XCD x the contiguous
+// run of clusters [x*n/8, (x+1)*n/8) in order, so that neighbouring clusters
+// (which share field-tile halos and J lines) meet in the same L2.  A
+// bijection of [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ inline int xcd_cluster(int b, int n)
+{
+    const int per = n / kXcds, rem = n % kXcds;
+    const int x = b % kXcds, k = b / kXcds;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 struct Cluster {
     int p;          // patch id
     int pc[3];      // patch coordinates

@@ -20,11 +20,12 @@ def make(precision, deposit=sa.DEPOSIT_ATOMIC, shape=(32, 32, 32), seed=9):
     return sim
 
 
-@pytest.mark.parametrize("threads", [64, 128, 512])
-def test_deterministic_form_independent_of_workgroup_size(threads):
+@pytest.mark.parametrize("threads,xcd", [(64, 0), (128, 1), (512, 0), (256, 1)])
+def test_deterministic_form_independent_of_launch_shape(threads, xcd):
     a = make(sa.PLAIN_FP64, sa.DEPOSIT_DETERMINISTIC)
     b = make(sa.PLAIN_FP64, sa.DEPOSIT_DETERMINISTIC)
     b.set_tuning(sa.TUNE_THREADS, threads)
+    b.set_tuning(sa.TUNE_XCD, xcd)
     a.step(5)
     b.step(5)
     for f in ("Ex", "Jy", "Bz_m"):
@@ -44,6 +45,7 @@ def test_autotune_keeps_state(precision):
     knobs = b.autotune(reps=2)
     assert knobs["threads"] in (128, 256, 512) and knobs["interleave"] in (0, 1)
     assert knobs["persistent"] in ((0, 1) if precision == sa.MIXED else (0,))
+    assert knobs["xcd"] in (0, 1) and not (knobs["xcd"] and knobs["persistent"])
     assert knobs["particles_ms"] > 0
     a.step(5)
     b.step(5)
@@ -117,3 +119,15 @@ def test_unordered_cells_hold_the_oracle_multisets():
         for k in ("x", "y", "z", "px", "py", "pz", "w"):
             scale = max(np.abs(b[k]).max(), 1e-300)
             assert np.abs(a[k][ia] - b[k][ib]).max() <= 1e-12 * scale, k
+
+
+def test_xcd_order_mixed_matches_grid_order():
+    a, b = make(sa.MIXED), make(sa.MIXED)
+    b.set_tuning(sa.TUNE_XCD, 1)
+    a.step(6)
+    b.step(6)
+    for s in range(2):
+        np.testing.assert_array_equal(a.get_particles(s)["key"], b.get_particles(s)["key"])
+    for f in ("Ex", "Jz"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
