@@ -68,8 +68,11 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
     using TL = Tile<ORDER>;
     constexpr int H = TL::H, GT = TL::GT, JT = TL::JT, S = TL::S;
     constexpr int GT3 = ND == 3 ? GT * GT * GT : GT * GT;
+    // 3D field tile x-stride: 68 doubles instead of 8*8, so that the four
+    // x-planes of cells a wavefront covers do not share LDS banks
+    constexpr int GSX = ND == 3 ? (ORDER == 2 ? 68 : GT * GT) : GT, GF = ND == 3 ? GT * GSX : GT3;
     constexpr int JT3 = ND == 3 ? JT * JT * JT : JT * JT;
-    __shared__ double sF[6 * GT3];
+    __shared__ double sF[6 * GF];
     __shared__ double sJ[DEPOSIT ? 3 * JT3 : 1];
     __shared__ int sHist[kMaxMerged][64];  // destination-cell counts of this cluster's own cells
     // 3D deposit: particles whose node changes on some axis are deposited in
@@ -104,8 +107,9 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
         const int gj = wrapi(cl.G0[1] - TL::GO + b, g.N[1]);
         const int gk = ND == 3 ? wrapi(cl.G0[2] - TL::GO + c, g.N[2]) : 0;
         const int64_t idx = ((int64_t)gi * g.N[1] + gj) * g.N[2] + gk;
+        const int so = ND == 3 ? a * GSX + b * GT + c : t;
 #pragma unroll
-        for (int f = 0; f < 6; ++f) sF[f * GT3 + t] = src[f][idx];
+        for (int f = 0; f < 6; ++f) sF[f * GF + so] = src[f][idx];
     }
     if constexpr (DEPOSIT)
         for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.;
@@ -188,19 +192,19 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
         }
         double Ep[3], Bp[3];
         if constexpr (ND == 3) {
-            Ep[0] = gather3<H, GT>(cd[0], cp[1], cp[2], sF + 0 * GT3, ld[0], lp[1], lp[2]);
-            Ep[1] = gather3<H, GT>(cp[0], cd[1], cp[2], sF + 1 * GT3, lp[0], ld[1], lp[2]);
-            Ep[2] = gather3<H, GT>(cp[0], cp[1], cd[2], sF + 2 * GT3, lp[0], lp[1], ld[2]);
-            Bp[0] = gather3<H, GT>(cp[0], cd[1], cd[2], sF + 3 * GT3, lp[0], ld[1], ld[2]);
-            Bp[1] = gather3<H, GT>(cd[0], cp[1], cd[2], sF + 4 * GT3, ld[0], lp[1], ld[2]);
-            Bp[2] = gather3<H, GT>(cd[0], cd[1], cp[2], sF + 5 * GT3, ld[0], ld[1], lp[2]);
+            Ep[0] = gather3<H, GT, GSX>(cd[0], cp[1], cp[2], sF + 0 * GF, ld[0], lp[1], lp[2]);
+            Ep[1] = gather3<H, GT, GSX>(cp[0], cd[1], cp[2], sF + 1 * GF, lp[0], ld[1], lp[2]);
+            Ep[2] = gather3<H, GT, GSX>(cp[0], cp[1], cd[2], sF + 2 * GF, lp[0], lp[1], ld[2]);
+            Bp[0] = gather3<H, GT, GSX>(cp[0], cd[1], cd[2], sF + 3 * GF, lp[0], ld[1], ld[2]);
+            Bp[1] = gather3<H, GT, GSX>(cd[0], cp[1], cd[2], sF + 4 * GF, ld[0], lp[1], ld[2]);
+            Bp[2] = gather3<H, GT, GSX>(cd[0], cd[1], cp[2], sF + 5 * GF, ld[0], ld[1], lp[2]);
         } else {
-            Ep[0] = gather2<H, GT>(cd[0], cp[1], sF + 0 * GT3, ld[0], lp[1]);
-            Ep[1] = gather2<H, GT>(cp[0], cd[1], sF + 1 * GT3, lp[0], ld[1]);
-            Ep[2] = gather2<H, GT>(cp[0], cp[1], sF + 2 * GT3, lp[0], lp[1]);
-            Bp[0] = gather2<H, GT>(cp[0], cd[1], sF + 3 * GT3, lp[0], ld[1]);
-            Bp[1] = gather2<H, GT>(cd[0], cp[1], sF + 4 * GT3, ld[0], lp[1]);
-            Bp[2] = gather2<H, GT>(cd[0], cd[1], sF + 5 * GT3, ld[0], ld[1]);
+            Ep[0] = gather2<H, GT>(cd[0], cp[1], sF + 0 * GF, ld[0], lp[1]);
+            Ep[1] = gather2<H, GT>(cp[0], cd[1], sF + 1 * GF, lp[0], ld[1]);
+            Ep[2] = gather2<H, GT>(cp[0], cp[1], sF + 2 * GF, lp[0], lp[1]);
+            Bp[0] = gather2<H, GT>(cp[0], cd[1], sF + 3 * GF, lp[0], ld[1]);
+            Bp[1] = gather2<H, GT>(cd[0], cp[1], sF + 4 * GF, ld[0], lp[1]);
+            Bp[2] = gather2<H, GT>(cd[0], cd[1], sF + 5 * GF, ld[0], ld[1]);
         }
 
         // ---- push

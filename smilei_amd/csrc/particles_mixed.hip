@@ -52,9 +52,10 @@ __device__ inline void splinef(float d, float *w)
     }
 }
 
-// Gather Sum_{a,b,c} X[a] Y[b] Z[c] f(i+a, j+b, k+c) from an LDS tile of
-// width T, contracted z first, then y, then x (3 FMA chains).
-template <int H, int T>
+// Gather Sum_{a,b,c} X[a] Y[b] Z[c] f(i+a, j+b, k+c) from an LDS tile with
+// strides SX, SY (z contiguous), contracted z first, then y, then x (3 FMA
+// chains).
+template <int H, int SX, int SY>
 __device__ inline float gatherf(const float *X, const float *Y, const float *Z, const float *t, int i, int j, int k)
 {
     float r = 0.f;
@@ -63,7 +64,7 @@ __device__ inline float gatherf(const float *X, const float *Y, const float *Z, 
         float ta = 0.f;
 #pragma unroll
         for (int b = -H; b <= H; ++b) {
-            const float *row = t + ((i + a) * T + (j + b)) * T + k;
+            const float *row = t + (i + a) * SX + (j + b) * SY + k;
             float sb = 0.f;
 #pragma unroll
             for (int c = -H; c <= H; ++c) sb = fmaf(Z[c + H], row[c], sb);
@@ -140,9 +141,9 @@ __device__ inline void windowd(double dold, double dnew, int sh, double *S0w, do
 }
 
 // Fixed-point accumulation of the J tile.  A contribution c, scaled to
-// units of the cluster's quantum, is rounded to an integer k by one FP64 add
-// of 1.5*2^52 (the sum's mantissa then holds k); the raw bits of that
-// double, 0x4338000000000000 + k, are summed with an integer LDS atomic.
+// units of the cluster's quantum, is rounded to an integer k in an FP64
+// number offset by 1.5*2^52 (whose mantissa then holds k); the raw bits of
+// that double, 0x4338000000000000 + k, are summed with an integer LDS atomic.
 // The constant only reaches bits 48..63, so the low 48 bits of a tile word
 // hold sum(k) exactly while |sum(k)| < 2^47.  The quantum of a cluster with
 // N particles is 2^-46 * N * cr_max (cr_max = the largest contribution of
@@ -160,8 +161,11 @@ __device__ inline double fixed_decode(unsigned long long v, double quantum)
 }
 
 // Windowed 3D Esirkepov with W(b,c) = S0b (S0c + DSc/2) + DSb (S0c/2 + DSc/3)
-// (the reference's four-term weight, factorised), FP64 weights and prefix
-// sums along the current's axis, fixed-point tile.  Only the non-zero part
+// (the reference's four-term weight, factorised, with the current factor
+// crq folded into the second row), FP64 weights and prefix sums along the
+// current's axis, fixed-point tile.  The prefix sum starts at the rounding
+// constant, so every step of it lands on an integer of the tile's quantum
+// and is added as it stands (one FMA per face).  Only the non-zero part
 // of the stencil is visited: NT window entries across the current (entries
 // past the union of the old and new supports are exact zeros) and NA faces
 // along it.  The face after the last window entry is left out: its prefix
@@ -180,15 +184,15 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
         double P[NT], Q[NT];
 #pragma unroll
         for (int m = 0; m < NT; ++m) {
-            P[m] = fma(0.5, DS[ex][m], S0[ex][m]);
-            Q[m] = fma(1.0 / 3.0, DS[ex][m], 0.5 * S0[ex][m]);
+            P[m] = crq[c] * fma(0.5, DS[ex][m], S0[ex][m]);
+            Q[m] = crq[c] * fma(1.0 / 3.0, DS[ex][m], 0.5 * S0[ex][m]);
         }
 #pragma unroll
         for (int a = 0; a < NT; ++a)
 #pragma unroll
             for (int b = 0; b < NT; ++b) {
-                const double t = crq[c] * fma(S0[bx][a], P[b], DS[bx][a] * Q[b]);
-                double acc = 0.;
+                const double t = fma(S0[bx][a], P[b], DS[bx][a] * Q[b]);
+                double acc = kRound;
 #pragma unroll
                 for (int r = 1; r <= NA; ++r) {
                     acc = fma(-DS[c][r - 1], t, acc);
@@ -196,8 +200,7 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
                     idx[c] = w0[c] + r;
                     idx[bx] = w0[bx] + a;
                     idx[ex] = w0[ex] + b;
-                    atomicAdd(tile(c, idx[0], idx[1], idx[2]),
-                              (unsigned long long)__double_as_longlong(acc + kRound));
+                    atomicAdd(tile(c, idx[0], idx[1], idx[2]), (unsigned long long)__double_as_longlong(acc));
                 }
             }
     }
@@ -213,8 +216,26 @@ __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, 
     constexpr int JT3 = JT * JT * JT;
     double S0[3][ORDER + 2], DS[3][ORDER + 2];
     int w0[3];
+    if constexpr (NT == ORDER + 1) {
+        // node unchanged on every axis: both supports start at window entry 0
 #pragma unroll
-    for (int d = 0; d < 3; ++d) windowd<ORDER>((double)dold[d], (double)dnew[d], sh[d], S0[d], DS[d], w0[d]);
+        for (int d = 0; d < 3; ++d) {
+            double s0[ORDER + 1], s1[ORDER + 1];
+            splined<ORDER>((double)dold[d], s0);
+            splined<ORDER>((double)dnew[d], s1);
+#pragma unroll
+            for (int m = 0; m <= ORDER; ++m) {
+                S0[d][m] = s0[m];
+                DS[d][m] = s1[m] - s0[m];
+            }
+            S0[d][ORDER + 1] = 0.;
+            DS[d][ORDER + 1] = 0.;
+            w0[d] = 1;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) windowd<ORDER>((double)dold[d], (double)dnew[d], sh[d], S0[d], DS[d], w0[d]);
+    }
     unsigned long long *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
     esirkepov3_fixed<ORDER, NT, NA>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
         return &sJb[c * JT3 + (a * JT + b) * JT + k];
@@ -265,7 +286,11 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     // J tile: stencil rows [c0-H-1, c0+4+H]
     constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
     constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
-    __shared__ float sF[6 * GT3];
+    // field tile strides: an x-stride of 76 floats (not 8*8 = 64, a multiple
+    // of the LDS bank count) spreads the four x-planes of cells that the
+    // lanes of a wavefront cover over different banks
+    constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GT * GSX;
+    __shared__ float sF[6 * GF];
     __shared__ unsigned long long sJ[3 * JT3];
     __shared__ int sHist[kMaxMerged][64];
     // particles whose node changes on some axis: their deposit (the wider
@@ -288,7 +313,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const int64_t idx = ((int64_t)wrapi(cl.G0[0] - GO + a, g.N[0]) * g.N[1] + wrapi(cl.G0[1] - GO + b, g.N[1])) *
                                 g.N[2] + wrapi(cl.G0[2] - GO + c, g.N[2]);
 #pragma unroll
-        for (int f = 0; f < 6; ++f) sF[f * GT3 + t] = (float)src[f][idx];
+        for (int f = 0; f < 6; ++f) sF[f * GF + a * GSX + b * GSY + c] = (float)src[f][idx];
     }
     for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0ull;
     for (int t = threadIdx.x; t < kMaxMerged * 64; t += blockDim.x) sHist[t / 64][t % 64] = 0;
@@ -346,12 +371,12 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const int p0 = l[0] + hi[0] + GO, p1 = l[1] + hi[1] + GO, p2 = l[2] + hi[2] + GO;
         const int d0 = l[0] + 1 + GO, d1 = l[1] + 1 + GO, d2 = l[2] + 1 + GO;
         float E[3], Bf[3];
-        E[0] = gatherf<H, GT>(cd[0], cp[1], cp[2], sF + 0 * GT3, d0, p1, p2);
-        E[1] = gatherf<H, GT>(cp[0], cd[1], cp[2], sF + 1 * GT3, p0, d1, p2);
-        E[2] = gatherf<H, GT>(cp[0], cp[1], cd[2], sF + 2 * GT3, p0, p1, d2);
-        Bf[0] = gatherf<H, GT>(cp[0], cd[1], cd[2], sF + 3 * GT3, p0, d1, d2);
-        Bf[1] = gatherf<H, GT>(cd[0], cp[1], cd[2], sF + 4 * GT3, d0, p1, d2);
-        Bf[2] = gatherf<H, GT>(cd[0], cd[1], cp[2], sF + 5 * GT3, d0, d1, p2);
+        E[0] = gatherf<H, GSX, GSY>(cd[0], cp[1], cp[2], sF + 0 * GF, d0, p1, p2);
+        E[1] = gatherf<H, GSX, GSY>(cp[0], cd[1], cp[2], sF + 1 * GF, p0, d1, p2);
+        E[2] = gatherf<H, GSX, GSY>(cp[0], cp[1], cd[2], sF + 2 * GF, p0, p1, d2);
+        Bf[0] = gatherf<H, GSX, GSY>(cp[0], cd[1], cd[2], sF + 3 * GF, p0, d1, d2);
+        Bf[1] = gatherf<H, GSX, GSY>(cd[0], cp[1], cd[2], sF + 4 * GF, d0, p1, d2);
+        Bf[2] = gatherf<H, GSX, GSY>(cd[0], cd[1], cp[2], sF + 5 * GF, d0, d1, p2);
 
         // ---- Boris push, FP32.  gamma from 1 + |u|^2 with the squares summed
         // before adding 1 and one Newton step on rsqrt.

@@ -109,7 +109,7 @@ __device__ inline void boris(double qmh, double dt, const double *E, const doubl
 }
 
 // Gather from an LDS tile of width T: sum_{a,b,c} ((cx*cy)*cz)*f, a outer.
-template <int H, int T>
+template <int H, int T, int SX = T * T>
 __device__ inline double gather3(const double *cx, const double *cy, const double *cz, const double *t,
                                  int i, int j, int k)
 {
@@ -120,7 +120,7 @@ __device__ inline double gather3(const double *cx, const double *cy, const doubl
         for (int b = -H; b <= H; ++b)
 #pragma unroll
             for (int c = -H; c <= H; ++c)
-                r += cx[a + H] * cy[b + H] * cz[c + H] * t[((i + a) * T + (j + b)) * T + (k + c)];
+                r += cx[a + H] * cy[b + H] * cz[c + H] * t[(i + a) * SX + (j + b) * T + (k + c)];
     return r;
 }
 
