@@ -243,7 +243,7 @@ __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, 
 }
 
 struct Cl {
-    int p, pc[3], G0[3];
+    int p, pc[3], G0[3], lc0[3];  // patch, its coordinates, first global / in-patch cell
     int64_t gbase;
 };
 
@@ -256,7 +256,10 @@ __device__ inline Cl cluster_of(const Geom &g, int b)
     c.pc[0] = c.p / (g.npatch[1] * g.npatch[2]);
     c.pc[1] = (c.p / g.npatch[2]) % g.npatch[1];
     c.pc[2] = c.p % g.npatch[2];
-    for (int d = 0; d < 3; ++d) c.G0[d] = c.pc[d] * g.n[d] + 4 * cc[d];
+    for (int d = 0; d < 3; ++d) {
+        c.lc0[d] = 4 * cc[d];
+        c.G0[d] = c.pc[d] * g.n[d] + c.lc0[d];
+    }
     c.gbase = (int64_t)c.p * g.cpp + (int64_t)cl * g.cells_per_cl;
     return c;
 }
@@ -403,26 +406,29 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         ig = rsqrtf(g2);
         ig = ig * fmaf(-0.5f * g2 * ig, ig, 1.5f);
 
-        // ---- move: offsets in cell units, exact integer cell change
+        // ---- move: offsets in cell units, exact integer cell change; the
+        // destination patch (periodic) and in-patch cell without division
         float on[3];
-        int cn[3], mdir[3], cw_[3];
+        int step[3], mdir[3], dpc[3], lp[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             float v = fmaf(un[d] * ig, dtdx[d], o[d]);
-            int step = 0;
+            int st = 0;
             if (v < 0.f) {
                 v += 1.f;
-                step = -1;
+                st = -1;
             } else if (v >= 1.f) {
                 v -= 1.f;
-                step = 1;
+                st = 1;
             }
             if (v >= 1.f) v = 0.99999994f;  // v+1 rounded up from a tiny negative
             on[d] = v;
-            cn[d] = cl.G0[d] + l[d] + step;  // unwrapped global cell
-            const int lo = cl.pc[d] * g.n[d];
-            mdir[d] = cn[d] < lo ? -1 : (cn[d] >= lo + g.n[d] ? 1 : 0);
-            cw_[d] = cn[d] < 0 ? cn[d] + g.N[d] : (cn[d] >= g.N[d] ? cn[d] - g.N[d] : cn[d]);
+            step[d] = st;
+            lp[d] = cl.lc0[d] + l[d] + st;  // in-patch cell, may leave the patch by one
+            mdir[d] = lp[d] < 0 ? -1 : (lp[d] >= g.n[d] ? 1 : 0);
+            lp[d] -= mdir[d] * g.n[d];
+            dpc[d] = cl.pc[d] + mdir[d];
+            dpc[d] = dpc[d] < 0 ? g.npatch[d] - 1 : (dpc[d] >= g.npatch[d] ? 0 : dpc[d]);
         }
 
         // ---- Esirkepov deposit into the FP32 LDS tile
@@ -432,7 +438,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int hn = on[d] >= 0.5f ? 1 : 0;
-                sh[d] = (cn[d] - cl.G0[d] - l[d]) + hn - hi[d];  // new node - old node
+                sh[d] = step[d] + hn - hi[d];  // new node - old node
                 moved |= sh[d];
                 dnew[d] = on[d] - (float)hn;
                 o3[d] = l[d] + hi[d];  // old node relative to G0 == tile coordinate of stencil index 0 (JO = H+1)
@@ -470,8 +476,10 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         }
 
         // ---- destination key, direction of motion, store
-        int dpc[3];
-        const uint32_t gkey = global_key3(g, cw_, dpc);
+        const int dp = (dpc[0] * g.npatch[1] + dpc[1]) * g.npatch[2] + dpc[2];
+        const int dcl = ((lp[0] >> 2) * g.ncl[1] + (lp[1] >> 2)) * g.ncl[2] + (lp[2] >> 2);
+        const uint32_t gkey =
+            (uint32_t)((int64_t)dp * g.cpp + dcl * 64 + 16 * (lp[0] & 3) + 4 * (lp[1] & 3) + (lp[2] & 3));
         const uint32_t dir = (uint32_t)((mdir[0] + 1) * 9 + (mdir[1] + 1) * 3 + (mdir[2] + 1));
         keysB[jslot] = (dir << 27) | gkey;
         B.ox[jslot] = on[0];
