@@ -111,6 +111,19 @@ int smi_set_particles(smi_context *ctx, int species, int64_t n,
  * cells, around the (y,z) axis centre; <= 0 disables) get no particles. */
 int smi_init_species(smi_context *ctx, int species, int ppc, double density,
                      double sigma, double drift_ux, double radius, uint64_t seed);
+/* Same with isotropic Maxwell-Juttner momenta of temperature theta =
+ * T / (m_species c^2) (momentum_initialization = "maxwell-juttner" of the
+ * reference, ParticleCreator.cpp:1002, in the rest frame; drift_ux is added
+ * to u_x), drawn by inverse transform from 4096 quantiles of |u| tabulated on
+ * the host; theta = 0 gives cold particles. */
+int smi_init_species_mj(smi_context *ctx, int species, int ppc, double density,
+                        double theta, double drift_ux, double radius, uint64_t seed);
+/* Particles at the positions of species `src` (weights times weight_scale),
+ * fresh Maxwell-Juttner momenta of temperature theta (0: cold) plus
+ * `drift_ux` along x: position_initialization = "<species name>" of the
+ * reference (src/Particles/ParticleCreator.cpp:277-284). */
+int smi_init_species_like(smi_context *ctx, int species, int src, double weight_scale, double theta,
+                          double drift_ux, uint64_t seed);
 /* Profile-driven initialisation, ParticleCreator::create
  * (src/Particles/ParticleCreator.cpp:151-346): host arrays [ncells] of the
  * number density and particles-per-cell profiles evaluated at the cell
@@ -121,6 +134,9 @@ int smi_init_species(smi_context *ctx, int species, int ppc, double density,
  * `drift` along x. */
 int smi_init_species_profile(smi_context *ctx, int species, const double *density, const double *ppc,
                              double sigma, double drift_ux, uint64_t seed);
+/* Same with Maxwell-Juttner momenta of temperature theta (as smi_init_species_mj). */
+int smi_init_species_profile_mj(smi_context *ctx, int species, const double *density, const double *ppc,
+                                double theta, double drift_ux, uint64_t seed);
 int64_t smi_num_particles(const smi_context *ctx, int species);
 /* Particles in canonical order (patch-major, then cell key, then arrival
  * order); patch and key receive the owning patch id ((px*npy+py)*npz+pz)

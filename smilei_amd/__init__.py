@@ -109,12 +109,30 @@ class Simulation:
                     persistent=self.get_tuning(TUNE_PERSISTENT), xcd=self.get_tuning(TUNE_XCD),
                     particles_ms=ms.value)
 
+    def init_species_mj(self, s, ppc, density, theta, drift_ux=0.0, radius=0.0, seed=0):
+        """Maxwell-Juttner momenta of temperature theta = T / (m c^2)."""
+        check(self._L.smi_init_species_mj(self.h, s, int(ppc), float(density), float(theta), float(drift_ux),
+                                          float(radius), int(seed)))
+
+    def init_species_like(self, s, src, weight_scale=1.0, theta=0.0, drift_ux=0.0, seed=0):
+        """Particles at the positions of species `src`, weights scaled,
+        fresh Maxwell-Juttner momenta of temperature theta (0: cold)."""
+        check(self._L.smi_init_species_like(self.h, s, src, float(weight_scale), float(theta), float(drift_ux),
+                                            int(seed)))
+
     def init_species_profile(self, s, density, ppc, sigma=0.0, drift_ux=0.0, seed=0):
         """Per-cell density and particles-per-cell arrays (shape of the grid,
         values at cell centres), ParticleCreator's rounding and weights."""
         d = np.ascontiguousarray(np.broadcast_to(density, self.shape), dtype=np.float64)
         n = np.ascontiguousarray(np.broadcast_to(ppc, self.shape), dtype=np.float64)
         check(self._L.smi_init_species_profile(self.h, s, _dp(d), _dp(n), float(sigma), float(drift_ux), int(seed)))
+
+    def init_species_profile_mj(self, s, density, ppc, theta, drift_ux=0.0, seed=0):
+        """Profiles as init_species_profile, Maxwell-Juttner momenta."""
+        d = np.ascontiguousarray(np.broadcast_to(density, self.shape), dtype=np.float64)
+        n = np.ascontiguousarray(np.broadcast_to(ppc, self.shape), dtype=np.float64)
+        check(self._L.smi_init_species_profile_mj(self.h, s, _dp(d), _dp(n), float(theta), float(drift_ux),
+                                                  int(seed)))
 
     def num_particles(self, s):
         return self._L.smi_num_particles(self.h, s)

@@ -45,6 +45,12 @@ SIGNATURES = {
     "smi_set_particles": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int64] + [_D] * 7),
     "smi_init_species": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+    "smi_init_species_mj": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+    "smi_init_species_profile_mj": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_uint64]),
+    "smi_init_species_like": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_uint64]),
     "smi_init_species_profile": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_uint64]),
     "smi_num_particles": (ctypes.c_int64, [_VP, ctypes.c_int]),

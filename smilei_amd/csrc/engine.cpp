@@ -40,6 +40,7 @@ struct SpeciesState {
     double *scratch = nullptr;     // 7*cap, deterministic mode
     SpeciesK k{};
     SpeciesKM km{};
+    double w_max = 0.;             // largest |weight| (fixed-point range of the mixed form)
 
     PartPtrs view(int b) const
     {
@@ -267,6 +268,7 @@ void set_quantum(const Geom &g, SpeciesState &s, double w_max)
         cell_volume *= g.dx[d];
         dxdt = std::max(dxdt, g.dx[d] / g.dt);
     }
+    s.w_max = w_max;
     double cmax = std::fabs(s.charge) / cell_volume * w_max * dxdt;
     if (!(cmax > 0)) cmax = 1.0;
     s.km.cr_max = cmax;
@@ -535,18 +537,79 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
 
 namespace {
 
+// Quantiles of |u| = |gamma beta| of the isotropic Maxwell-Juttner
+// distribution f(u) ~ u^2 exp(-(gamma - 1) / theta), theta = T / (m c^2):
+// the CDF integrated (Simpson) on a fine grid up to gamma - 1 = 60 theta
+// (tail below e^-50), inverted at nq equally spaced probabilities.
+std::vector<double> mj_quantiles(double theta, int nq)
+{
+    const int nf = 1 << 16;  // even number of intervals
+    const double gm = 1.0 + 60.0 * theta, umax = std::sqrt(gm * gm - 1.0);
+    std::vector<double> u(nf + 1), f(nf + 1), cdf(nf + 1, 0.0);
+    for (int i = 0; i <= nf; ++i) {
+        u[i] = umax * i / nf;
+        const double gam1 = u[i] * u[i] / (std::sqrt(1.0 + u[i] * u[i]) + 1.0);  // gamma - 1 without cancellation
+        f[i] = u[i] * u[i] * std::exp(-gam1 / theta);
+    }
+    const double h = umax / nf;
+    for (int i = 2; i <= nf; i += 2) {
+        cdf[i] = cdf[i - 2] + h / 3.0 * (f[i - 2] + 4.0 * f[i - 1] + f[i]);
+        cdf[i - 1] = cdf[i - 2] + h / 2.0 * (f[i - 2] + f[i - 1]);  // trapezoid at the odd point
+    }
+    const double tot = cdf[nf];
+    std::vector<double> q(nq);
+    int j = 0;
+    for (int k = 0; k < nq; ++k) {
+        const double target = tot * k / (nq - 1);
+        while (j < nf && cdf[j + 1] < target) ++j;
+        if (j >= nf) {
+            q[k] = umax;
+            continue;
+        }
+        const double span = cdf[j + 1] - cdf[j];
+        q[k] = u[j] + (span > 0 ? (target - cdf[j]) / span : 0.0) * h;
+    }
+    q[0] = 0.0;
+    q[nq - 1] = umax;
+    return q;
+}
+
+constexpr int kMjQuantiles = 4096;
+
+// Momentum distribution of an initialiser: Gaussian components of std sigma,
+// or Maxwell-Juttner of temperature theta (m c^2 units) when theta > 0.
+struct Momenta {
+    double sigma = 0., theta = 0., drift = 0.;
+};
+
+// Device table of the Maxwell-Juttner quantiles (or nothing for Gaussian).
+struct MjTable {
+    DevBuf<double> buf;
+    int n = 0;
+    explicit MjTable(const Momenta &m)
+    {
+        if (!(m.theta > 0)) return;
+        const std::vector<double> q = mj_quantiles(m.theta, kMjQuantiles);
+        buf.alloc(kMjQuantiles);
+        SMI_HIP(hipMemcpy(buf.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+        n = kMjQuantiles;
+    }
+};
+
 // Shared tail of the initialisers: draw n particles (particle i in cell
 // cells[i / ppc]), bin and sort them.  cells/weights are device arrays or
 // null; w_max bounds |weight| (fixed-point range of the mixed form).
 void init_particles(smi_context *c, int s, int64_t n, int ppc, const int32_t *d_cells, const double *d_weights,
-                    double weight, double w_max, double sigma, double drift_ux, uint64_t seed)
+                    double weight, double w_max, const Momenta &mom, uint64_t seed)
 {
     SpeciesState &S = c->sp[s];
     const Geom &g = c->g;
     const MixPtrs none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (c->mixed) set_quantum(g, S, w_max);
+    MjTable mj(mom);
     launch_init(g, S.view(c->cur), c->mixed ? S.mview(c->cur) : none, S.keys[c->cur], n, ppc, d_cells, d_weights,
-                weight, sigma, drift_ux, seed * 0x100000001B3ull + (uint64_t)s, c->st);
+                weight, mom.sigma, mj.n ? mj.buf.p : nullptr, mj.n, mom.drift, seed * 0x100000001B3ull + (uint64_t)s,
+                c->st);
     S.n = n;
     if (!c->mixed) launch_bin_initial(g, S.view(c->cur), n, S.keys[c->cur], c->st);
     launch_sort(g, S.keys[c->cur], n, S.sb, c->st);
@@ -562,11 +625,10 @@ double cell_volume_of(const Geom &g)
 
 }  // namespace
 
-int smi_init_species(smi_context *c, int s, int ppc, double density, double sigma, double drift_ux, double radius,
-                     uint64_t seed)
+namespace {
+
+void init_constant(smi_context *c, int s, int ppc, double density, const Momenta &mom, double radius, uint64_t seed)
 {
-    SMI_TRY
-    check_ctx(c);
     if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
     if (ppc <= 0) throw Error(SMI_ERR_INVALID, "ppc must be > 0");
     SpeciesState &S = c->sp[s];
@@ -594,17 +656,75 @@ int smi_init_species(smi_context *c, int s, int ppc, double density, double sigm
         SMI_HIP(hipMemcpy(d_cells.p, act.data(), act.size() * 4, hipMemcpyHostToDevice));
     }
     const double weight = density * cell_volume_of(g) / ppc;
-    init_particles(c, s, n, ppc, radius > 0 ? d_cells.p : nullptr, nullptr, weight, std::fabs(weight), sigma,
-                   drift_ux, seed);
+    init_particles(c, s, n, ppc, radius > 0 ? d_cells.p : nullptr, nullptr, weight, std::fabs(weight), mom, seed);
+}
+
+}  // namespace
+
+int smi_init_species(smi_context *c, int s, int ppc, double density, double sigma, double drift_ux, double radius,
+                     uint64_t seed)
+{
+    SMI_TRY
+    check_ctx(c);
+    Momenta m;
+    m.sigma = sigma;
+    m.drift = drift_ux;
+    init_constant(c, s, ppc, density, m, radius, seed);
     return SMI_OK;
     SMI_CATCH
 }
 
-int smi_init_species_profile(smi_context *c, int s, const double *density, const double *ppc, double sigma,
-                             double drift_ux, uint64_t seed)
+int smi_init_species_mj(smi_context *c, int s, int ppc, double density, double theta, double drift_ux, double radius,
+                        uint64_t seed)
 {
     SMI_TRY
     check_ctx(c);
+    if (!(theta >= 0) || !std::isfinite(theta)) throw Error(SMI_ERR_INVALID, "theta must be finite and >= 0");
+    Momenta m;
+    m.theta = theta;
+    m.drift = drift_ux;
+    init_constant(c, s, ppc, density, m, radius, seed);
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_init_species_like(smi_context *c, int s, int src, double weight_scale, double theta, double drift_ux,
+                          uint64_t seed)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size() || src < 0 || src >= (int)c->sp.size())
+        throw Error(SMI_ERR_INVALID, "bad species id");
+    if (s == src) throw Error(SMI_ERR_INVALID, "a species cannot take its positions from itself");
+    if (!std::isfinite(weight_scale)) throw Error(SMI_ERR_INVALID, "weight_scale must be finite");
+    if (!(theta >= 0) || !std::isfinite(theta)) throw Error(SMI_ERR_INVALID, "theta must be finite and >= 0");
+    SpeciesState &S = c->sp[s];
+    const SpeciesState &R = c->sp[src];
+    if (R.n > S.cap) throw Error(SMI_ERR_CAPACITY, "the source species has more particles than this capacity");
+    SMI_HIP(hipSetDevice(c->p.device));
+    drop_graphs(c);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    const MixPtrs none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const PartPtrs pnone{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (c->mixed) set_quantum(c->g, S, R.w_max * std::fabs(weight_scale));
+    Momenta mom;
+    mom.theta = theta;
+    MjTable mj(mom);
+    launch_init_like(c->g, c->mixed ? pnone : R.view(c->cur), c->mixed ? R.mview(c->cur) : none, R.keys[c->cur],
+                     c->mixed ? pnone : S.view(c->cur), c->mixed ? S.mview(c->cur) : none, S.keys[c->cur], R.n,
+                     weight_scale, 0.0, mj.n ? mj.buf.p : nullptr, mj.n, drift_ux,
+                     seed * 0x100000001B3ull + (uint64_t)s, c->st);
+    S.n = R.n;
+    launch_sort(c->g, S.keys[c->cur], S.n, S.sb, c->st);
+    SMI_HIP(hipStreamSynchronize(c->st));
+    return SMI_OK;
+    SMI_CATCH
+}
+
+namespace {
+
+void init_profile(smi_context *c, int s, const double *density, const double *ppc, const Momenta &mom, uint64_t seed)
+{
     if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
     if (!density || !ppc) throw Error(SMI_ERR_INVALID, "density and ppc arrays are required");
     SpeciesState &S = c->sp[s];
@@ -653,7 +773,34 @@ int smi_init_species_profile(smi_context *c, int s, const double *density, const
         SMI_HIP(hipMemcpy(d_cells.p, cells.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
         SMI_HIP(hipMemcpy(d_w.p, wts.data(), n * sizeof(double), hipMemcpyHostToDevice));
     }
-    init_particles(c, s, n, 1, d_cells.p, d_w.p, 0., w_max, sigma, drift_ux, seed);
+    init_particles(c, s, n, 1, d_cells.p, d_w.p, 0., w_max, mom, seed);
+}
+
+}  // namespace
+
+int smi_init_species_profile(smi_context *c, int s, const double *density, const double *ppc, double sigma,
+                             double drift_ux, uint64_t seed)
+{
+    SMI_TRY
+    check_ctx(c);
+    Momenta m;
+    m.sigma = sigma;
+    m.drift = drift_ux;
+    init_profile(c, s, density, ppc, m, seed);
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_init_species_profile_mj(smi_context *c, int s, const double *density, const double *ppc, double theta,
+                                double drift_ux, uint64_t seed)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (!(theta >= 0) || !std::isfinite(theta)) throw Error(SMI_ERR_INVALID, "theta must be finite and >= 0");
+    Momenta m;
+    m.theta = theta;
+    m.drift = drift_ux;
+    init_profile(c, s, density, ppc, m, seed);
     return SMI_OK;
     SMI_CATCH
 }

@@ -28,11 +28,35 @@ __device__ inline double u01(uint64_t seed, uint64_t i, uint32_t k)
     return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+// Momentum of particle i: with a quantile table uq[0..nq) of |u| (a
+// Maxwell-Juttner distribution tabulated on the host) an isotropic vector of
+// that length, otherwise Gaussian components of std sigma; plus drift on x.
+__device__ inline void draw_momentum(uint64_t seed, int64_t i, double sigma, const double *uq, int nq, double drift,
+                                     double *u)
+{
+    if (uq) {
+        const double r = u01(seed, i, 3) * (nq - 1);
+        const int k = min((int)r, nq - 2);
+        const double mag = uq[k] + (r - k) * (uq[k + 1] - uq[k]);
+        const double ct = 2.0 * u01(seed, i, 4) - 1.0, st = sqrt(fmax(0.0, 1.0 - ct * ct));
+        const double ph = 6.283185307179586 * u01(seed, i, 5);
+        u[0] = mag * st * cos(ph);
+        u[1] = mag * st * sin(ph);
+        u[2] = mag * ct;
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            const double r1 = u01(seed, i, 3 + 2 * d), r2 = u01(seed, i, 4 + 2 * d);
+            u[d] = sigma * sqrt(-2.0 * log(r1)) * cos(6.283185307179586 * r2);
+        }
+    }
+    u[0] += drift;
+}
+
 // Particle i lies in cell cells[i / ppc] (all cells in order when cells is
 // null) and has weight weights[i] (or `weight` when weights is null).
 __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ keys, int64_t n, int ppc,
                        const int32_t *__restrict__ cells, const double *__restrict__ weights, double weight,
-                       double sigma, double drift, uint64_t seed)
+                       double sigma, const double *__restrict__ uq, int nq, double drift, uint64_t seed)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -50,10 +74,7 @@ __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ key
         x[d] = v;
     }
     double u[3];
-    for (int d = 0; d < 3; ++d) {
-        const double r1 = u01(seed, i, 3 + 2 * d), r2 = u01(seed, i, 4 + 2 * d);
-        u[d] = sigma * sqrt(-2.0 * log(r1)) * cos(6.283185307179586 * r2);
-    }
+    draw_momentum(seed, i, sigma, uq, nq, drift, u);
     if (M.ox) {
         // split-precision form: exact cell, FP32 offset of the same draw
         int pc[3], l[3];
@@ -72,7 +93,7 @@ __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ key
         M.ox[i] = fo[0];
         M.oy[i] = fo[1];
         M.oz[i] = fo[2];
-        M.ux[i] = (float)(u[0] + drift);
+        M.ux[i] = (float)u[0];
         M.uy[i] = (float)u[1];
         M.uz[i] = (float)u[2];
         M.w[i] = (float)weight;
@@ -81,21 +102,63 @@ __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ key
     A.x[i] = x[0];
     A.y[i] = x[1];
     A.z[i] = g.nd == 3 ? x[2] : 0.;
-    A.px[i] = u[0] + drift;
+    A.px[i] = u[0];
     A.py[i] = u[1];
     A.pz[i] = u[2];
     A.w[i] = weight;
 }
 
+// Particles at the positions (and, scaled, the weights) of an existing
+// species, fresh Gaussian momenta: Smilei's position_initialization = "<other
+// species>" (ParticleCreator.cpp:277-284).  Buffer order is kept; keys are
+// the source's cells with direction 13.
+__global__ void k_init_like(PartPtrs S, MixPtrs SM, const uint32_t *__restrict__ skeys, PartPtrs D, MixPtrs DM,
+                            uint32_t *__restrict__ dkeys, int64_t n, int nd, double wscale, double sigma,
+                            const double *__restrict__ uq, int nq, double drift, uint64_t seed)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[3];
+    draw_momentum(seed, i, sigma, uq, nq, drift, u);
+    dkeys[i] = (13u << 27) | (skeys[i] & ((1u << 27) - 1));
+    if (DM.ox) {
+        DM.ox[i] = SM.ox[i];
+        DM.oy[i] = SM.oy[i];
+        DM.oz[i] = SM.oz[i];
+        DM.ux[i] = (float)u[0];
+        DM.uy[i] = (float)u[1];
+        DM.uz[i] = (float)u[2];
+        DM.w[i] = (float)((double)SM.w[i] * wscale);
+        return;
+    }
+    D.x[i] = S.x[i];
+    D.y[i] = S.y[i];
+    D.z[i] = nd == 3 ? S.z[i] : 0.;
+    D.px[i] = u[0];
+    D.py[i] = u[1];
+    D.pz[i] = u[2];
+    D.w[i] = S.w[i] * wscale;
+}
+
 }  // namespace
 
+void launch_init_like(const Geom &g, const PartPtrs &S, const MixPtrs &SM, const uint32_t *skeys, const PartPtrs &D,
+                      const MixPtrs &DM, uint32_t *dkeys, int64_t n, double wscale, double sigma, const double *uq,
+                      int nq, double drift, uint64_t seed, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_init_like, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S, SM, skeys, D, DM, dkeys, n,
+                       g.nd, wscale, sigma, uq, nq, drift, seed);
+    SMI_LAUNCH_CHECK();
+}
+
 void launch_init(const Geom &g, const PartPtrs &A, const MixPtrs &M, uint32_t *keys, int64_t n, int ppc,
-                 const int32_t *cells, const double *weights, double weight, double sigma, double drift,
-                 uint64_t seed, hipStream_t st)
+                 const int32_t *cells, const double *weights, double weight, double sigma, const double *uq, int nq,
+                 double drift, uint64_t seed, hipStream_t st)
 {
     if (n == 0) return;
     hipLaunchKernelGGL(k_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, A, M, keys, n, ppc, cells,
-                       weights, weight, sigma, drift, seed);
+                       weights, weight, sigma, uq, nq, drift, seed);
     SMI_LAUNCH_CHECK();
 }
 

@@ -131,9 +131,14 @@ void launch_rho_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &M, cons
 // Particle i lives in cell cells[i/ppc] (or cell i/ppc when cells == null).
 // Writes FP64 particles to A, or (when M.ox != null) mixed particles to M
 // plus their cell keys (direction 13) to `keys`.
+// Momenta: Gaussian components of std sigma, or, with a table uq[0..nq) of
+// |u| quantiles, an isotropic Maxwell-Juttner draw; plus drift along x.
+void launch_init_like(const Geom &g, const PartPtrs &S, const MixPtrs &SM, const uint32_t *skeys, const PartPtrs &D,
+                      const MixPtrs &DM, uint32_t *dkeys, int64_t n, double wscale, double sigma, const double *uq,
+                      int nq, double drift, uint64_t seed, hipStream_t st);
 void launch_init(const Geom &g, const PartPtrs &A, const MixPtrs &M, uint32_t *keys, int64_t n, int ppc,
-                 const int32_t *cells, const double *weights, double weight, double sigma, double drift,
-                 uint64_t seed, hipStream_t st);
+                 const int32_t *cells, const double *weights, double weight, double sigma, const double *uq, int nq,
+                 double drift, uint64_t seed, hipStream_t st);
 
 // ---- fields (fields.hip) ----------------------------------------------
 struct FieldSet {

@@ -1,0 +1,293 @@
+"""Namelist front-end: run a Smilei-style namelist on this library.
+
+A Smilei namelist is a Python script that calls ``Main(...)``,
+``Species(...)``, ``ExternalField(...)``, ``DiagScalar(...)`` and other
+blocks (src/Python/pyinit.py:156-640).  ``load`` executes such a script with
+this module's block classes in its namespace and checks the blocks against
+what the hot path implements; ``build`` turns the result into a
+``Simulation`` (species created in namelist order, external fields
+applied); ``run`` steps it for the namelist's number of time steps and
+records the scalar diagnostics.
+
+Supported (the periodic, Yee, Boris, momentum-conserving hot path):
+
+* ``Main``: geometry ("2Dcartesian", "3Dcartesian"), interpolation_order
+  (2, 4), two of cell_length / grid_length / number_of_cells,
+  number_of_patches, timestep or timestep_over_CFL, simulation_time or
+  number_of_timesteps (``n_time = int(simulation_time / timestep)``,
+  Params.cpp:1154), periodic EM_boundary_conditions, random_seed;
+* ``Species``: mass, charge, particles_per_cell, number_density or
+  charge_density (numbers or profiles of the position), temperature in
+  m_e c^2 (one value or three equal values), mean_velocity along x (added
+  to u_x as gamma v), momentum_initialization "maxwell-juttner"/"mj"
+  (isotropic Maxwell-Juttner of temperature T/m, drawn exactly by the
+  library) or "cold", position_initialization "random" or the name of an
+  earlier species (copy of its positions), periodic boundary_conditions,
+  pusher "boris";
+* ``ExternalField`` (initial fields, applied as applyExternalFields does);
+* ``DiagScalar`` (Uelm, Ukin and per-species Ukin every ``every`` steps).
+
+Blocks that change nothing on this path (load balancing, vectorisation,
+checkpoints and the other diagnostics) are accepted and recorded; blocks
+that would change the physics (lasers, moving window, collisions,
+injectors, filters, antennas, walls, radiation) raise ``NamelistError``
+rather than run a different simulation.
+"""
+import math
+
+import numpy as np
+
+from . import MIXED, PLAIN_FP64, DEPOSIT_ATOMIC, Simulation
+from . import profiles as P
+
+
+class NamelistError(Exception):
+    pass
+
+
+class _Block:
+    """A namelist block: its keyword arguments as attributes."""
+    defaults = {}
+
+    def __init__(self, **kw):
+        for k, v in self.defaults.items():
+            setattr(self, k, v)
+        for k, v in kw.items():
+            setattr(self, k, v)
+        self._kw = kw
+
+
+class _Main(_Block):
+    defaults = dict(geometry=None, interpolation_order=2, interpolator="momentum-conserving", cell_length=[],
+                    grid_length=[], number_of_cells=[], number_of_patches=None, timestep=None,
+                    timestep_over_CFL=None, simulation_time=None, number_of_timesteps=None,
+                    EM_boundary_conditions=[["periodic"]], maxwell_solver="Yee", random_seed=None,
+                    print_every=None, solve_poisson=True, gpu_computing=False)
+
+
+class _Species(_Block):
+    defaults = dict(name=None, mass=None, charge=None, particles_per_cell=None, number_density=None,
+                    charge_density=None, temperature=[1e-10], mean_velocity=[0.0, 0.0, 0.0],
+                    momentum_initialization="cold", position_initialization="random", pusher="boris",
+                    boundary_conditions=[["periodic"]], time_frozen=0.0, c_part_max=1.0, is_test=False,
+                    ionization_model="none", radiation_model="none")
+
+
+class _ExternalField(_Block):
+    defaults = dict(field=None, profile=None)
+
+
+class _DiagScalar(_Block):
+    defaults = dict(every=1, vars=[], precision=10)
+
+
+_IGNORED = ("LoadBalancing", "Vectorization", "Checkpoints", "DiagFields", "DiagProbe", "DiagParticleBinning",
+            "DiagScreen", "DiagTrackParticles", "DiagPerformances", "DiagRadiationSpectrum", "DiagNewParticles",
+            "MultipleDecomposition")
+_REFUSED = ("Laser", "LaserEnvelope", "LaserGaussian2D", "LaserGaussian3D", "LaserPlanar1D", "LaserOffset",
+            "LaserGaussianAM", "MovingWindow", "Collisions", "ParticleInjector", "PrescribedField", "Antenna",
+            "PartWall", "RadiationReaction", "MultiphotonBreitWheeler", "CurrentFilter", "FieldFilter",
+            "SmileiSingleton")
+
+
+class Namelist:
+    """Blocks collected from one namelist."""
+
+    def __init__(self):
+        self.main = None
+        self.species = []
+        self.external_fields = []
+        self.diag_scalar = None
+        self.ignored = []
+
+    # ---- derived parameters (after validation)
+    @property
+    def ndim(self):
+        return 2 if self.main.geometry == "2Dcartesian" else 3
+
+    def _namespace(self):
+        nl = self
+
+        def Main(**kw):
+            if nl.main is not None:
+                raise NamelistError("Main defined twice")
+            nl.main = _Main(**kw)
+
+        def Species(**kw):
+            nl.species.append(_Species(**kw))
+
+        def ExternalField(**kw):
+            nl.external_fields.append(_ExternalField(**kw))
+
+        def DiagScalar(**kw):
+            nl.diag_scalar = _DiagScalar(**kw)
+
+        ns = {"Main": Main, "Species": Species, "ExternalField": ExternalField, "DiagScalar": DiagScalar,
+              "math": math, "__name__": "__namelist__"}
+        for name in _IGNORED:
+            ns[name] = (lambda name: lambda *a, **kw: nl.ignored.append(name))(name)
+        for name in _REFUSED:
+            def refuse(*a, _name=name, **kw):
+                raise NamelistError("%s is not part of this library's hot path (periodic, Yee, Boris, "
+                                    "Esirkepov); the namelist cannot run here" % _name)
+            ns[name] = refuse
+        return ns
+
+    def validate(self):
+        m = self.main
+        if m is None:
+            raise NamelistError("the namelist defines no Main block")
+        if m.geometry not in ("2Dcartesian", "3Dcartesian"):
+            raise NamelistError("Main.geometry %r: only 2Dcartesian and 3Dcartesian are supported" % m.geometry)
+        nd = self.ndim
+        given = [len(m.cell_length) > 0, len(m.grid_length) > 0, len(m.number_of_cells) > 0]
+        if sum(given) != 2:
+            raise NamelistError("Main: define exactly two of grid_length, number_of_cells and cell_length")
+        if not given[2]:
+            m.number_of_cells = [int(round(L / d)) for L, d in zip(m.grid_length, m.cell_length)]
+        elif not given[0]:
+            m.cell_length = [L / n for L, n in zip(m.grid_length, m.number_of_cells)]
+        m.grid_length = [n * d for n, d in zip(m.number_of_cells, m.cell_length)]
+        if len(m.cell_length) != nd or len(m.number_of_cells) != nd:
+            raise NamelistError("Main: cell_length / number_of_cells need %d values" % nd)
+        if m.number_of_patches is None or len(m.number_of_patches) != nd:
+            raise NamelistError("Main.number_of_patches needs %d values" % nd)
+        for n, p in zip(m.number_of_cells, m.number_of_patches):
+            if p <= 0 or n % p:
+                raise NamelistError("Main: number_of_patches must divide the number of cells")
+        if m.timestep is None:
+            if m.timestep_over_CFL is None:
+                raise NamelistError("Main: timestep and timestep_over_CFL not defined")
+            m.timestep = m.timestep_over_CFL / math.sqrt(sum(1.0 / d ** 2 for d in m.cell_length))
+        if m.number_of_timesteps is None:
+            if m.simulation_time is None:
+                raise NamelistError("Main: simulation_time and number_of_timesteps not defined")
+            m.number_of_timesteps = int(m.simulation_time / m.timestep)
+        for bc in m.EM_boundary_conditions:
+            if any(b != "periodic" for b in bc):
+                raise NamelistError("Main.EM_boundary_conditions: only periodic boundaries are supported")
+        names = []
+        for s in self.species:
+            if s.name is None or s.mass is None or s.charge is None:
+                raise NamelistError("Species needs name, mass and charge")
+            if s.particles_per_cell is None:
+                raise NamelistError("Species %s: particles_per_cell not defined" % s.name)
+            if (s.number_density is None) == (s.charge_density is None):
+                raise NamelistError("Species %s: define one of number_density and charge_density" % s.name)
+            for bc in s.boundary_conditions:
+                if any(b != "periodic" for b in bc):
+                    raise NamelistError("Species %s: only periodic boundary conditions are supported" % s.name)
+            if s.pusher != "boris":
+                raise NamelistError("Species %s: pusher %r is not supported (boris only)" % (s.name, s.pusher))
+            if s.time_frozen and s.time_frozen > 0:
+                raise NamelistError("Species %s: frozen species are not supported" % s.name)
+            if s.ionization_model != "none" or s.radiation_model != "none":
+                raise NamelistError("Species %s: ionisation / radiation are not supported" % s.name)
+            mv = list(s.mean_velocity) + [0.0] * (3 - len(s.mean_velocity))
+            if callable(mv[0]) or any(callable(v) for v in mv) or mv[1] != 0 or mv[2] != 0:
+                raise NamelistError("Species %s: mean_velocity must be a constant along x" % s.name)
+            if not abs(mv[0]) < 1:
+                raise NamelistError("Species %s: |mean_velocity| must be below 1 (c)" % s.name)
+            T = list(s.temperature)
+            if any(callable(t) for t in T) or len(set(T)) != 1:
+                raise NamelistError("Species %s: temperature must be one constant (isotropic)" % s.name)
+            mi = s.momentum_initialization
+            if mi not in ("maxwell-juttner", "mj", "cold"):
+                raise NamelistError("Species %s: momentum_initialization %r not supported" % (s.name, mi))
+            pi = s.position_initialization
+            if pi != "random" and pi not in names:
+                raise NamelistError("Species %s: position_initialization %r (random or an earlier species name)"
+                                    % (s.name, pi))
+            names.append(s.name)
+        for e in self.external_fields:
+            if e.field not in P.DUAL or e.field.startswith("J"):
+                raise NamelistError("ExternalField: field %r is not an E, B or B_m component" % e.field)
+        return self
+
+
+def load(source):
+    """Execute a namelist (a file path, or the script text when it contains
+    a newline) and return its validated blocks."""
+    if "\n" not in source:
+        with open(source) as f:
+            source = f.read()
+    nl = Namelist()
+    exec(compile(source, "<namelist>", "exec"), nl._namespace())
+    return nl.validate()
+
+
+def _density(s):
+    if s.number_density is not None:
+        return s.number_density
+    q = abs(float(s.charge))
+    cd = s.charge_density
+    return (lambda x, y, z: np.abs(np.asarray(cd(x, y, z), dtype=float)) / q) if callable(cd) else abs(cd) / q
+
+
+def build(nl, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True):
+    """Create the Simulation of a loaded namelist: species in namelist order,
+    then the external fields."""
+    m = nl.main
+    patch = [n // p for n, p in zip(m.number_of_cells, m.number_of_patches)]
+    sim = Simulation(m.geometry, m.interpolation_order, m.number_of_cells, patch, m.cell_length, m.timestep,
+                     precision=precision, deposit=deposit, use_graph=use_graph, interpolator=m.interpolator,
+                     maxwell_solver=m.maxwell_solver)
+    seed = int(m.random_seed or 0)
+    ids = {}
+    ncell = int(np.prod(m.number_of_cells))
+    for k, s in enumerate(nl.species):
+        dens = _density(s)
+        theta = float(s.temperature[0]) / float(s.mass) if s.momentum_initialization != "cold" else 0.0
+        v = float(s.mean_velocity[0])
+        drift = v / math.sqrt(1.0 - v * v)
+        ppc = s.particles_per_cell
+        if s.position_initialization != "random":
+            src = ids[s.position_initialization]
+            cap = sim.num_particles(src)
+            sid = sim.add_species(s.name, s.mass, s.charge, cap)
+            src_dens = _density(next(b for b in nl.species if b.name == s.position_initialization))
+            if callable(dens) or callable(src_dens):
+                raise NamelistError("Species %s: copied positions need constant densities" % s.name)
+            sim.init_species_like(sid, src, weight_scale=dens / src_dens if src_dens else 0.0, theta=theta,
+                                  drift_ux=drift, seed=seed + 1 + k)
+        elif not callable(dens) and not callable(ppc) and float(ppc) == int(ppc) and dens > 0:
+            sid = sim.add_species(s.name, s.mass, s.charge, ncell * int(ppc))
+            sim.init_species_mj(sid, int(ppc), dens, theta, drift_ux=drift, seed=seed + 1 + k)
+        else:
+            cap = P.count_particles(sim, ppc, dens)
+            sid = sim.add_species(s.name, s.mass, s.charge, max(cap, 1))
+            P.create_species(sim, sid, dens, ppc, theta=theta, drift_ux=drift, seed=seed + 1 + k)
+        ids[s.name] = sid
+    if nl.external_fields:
+        P.apply_external_fields(sim, {e.field: e.profile for e in nl.external_fields})
+    return sim
+
+
+def run(source, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, steps=None, scalars_path=None):
+    """Load, build and run a namelist for its number of time steps (or
+    `steps`); returns (simulation, scalars) where scalars is an array of rows
+    (time, Utot, Ukin, Uelm, Ukin_<species>...) every DiagScalar.every
+    steps (every step without a DiagScalar block), also written as text to
+    `scalars_path` when given."""
+    nl = load(source)
+    sim = build(nl, precision=precision, deposit=deposit)
+    n = nl.main.number_of_timesteps if steps is None else int(steps)
+    every = max(int(nl.diag_scalar.every), 1) if nl.diag_scalar is not None else 1
+    rows = []
+
+    def record(it):
+        e = sim.energies()
+        rows.append([it * nl.main.timestep, float(e.sum()), float(e[1:].sum()), float(e[0])] + list(e[1:]))
+
+    record(0)
+    done = 0
+    while done < n:
+        k = min(every, n - done)
+        sim.step(k)
+        done += k
+        record(done)
+    rows = np.array(rows)
+    if scalars_path:
+        head = "time Utot Ukin Uelm " + " ".join("Ukin_%s" % s.name for s in nl.species)
+        np.savetxt(scalars_path, rows, header=head)
+    return sim, rows

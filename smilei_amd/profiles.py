@@ -51,13 +51,17 @@ def evaluate(profile, x, y, z, shape):
     return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), shape))
 
 
-def create_species(sim, sid, density, particles_per_cell, sigma=0.0, drift_ux=0.0, seed=0):
+def create_species(sim, sid, density, particles_per_cell, sigma=0.0, drift_ux=0.0, seed=0, theta=None):
     """Species creation from density and particles-per-cell profiles (each
-    a callable of the cell-centre positions, or a number)."""
+    a callable of the cell-centre positions, or a number); momenta Gaussian
+    of std sigma, or Maxwell-Juttner of temperature theta when given."""
     x, y, z = cell_centres(sim)
     n = evaluate(density, x, y, z, sim.shape)
     ppc = evaluate(particles_per_cell, x, y, z, sim.shape)
-    sim.init_species_profile(sid, n, ppc, sigma=sigma, drift_ux=drift_ux, seed=seed)
+    if theta is not None:
+        sim.init_species_profile_mj(sid, n, ppc, theta, drift_ux=drift_ux, seed=seed)
+    else:
+        sim.init_species_profile(sid, n, ppc, sigma=sigma, drift_ux=drift_ux, seed=seed)
     return n, ppc
 
 
