@@ -44,6 +44,13 @@ class Simulation:
                  precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True, device=0,
                  interpolator="momentum-conserving", pusher="boris", maxwell_solver="Yee"):
         self._L = _lib.load()
+        self._args = dict(geometry=geometry, interpolation_order=int(interpolation_order),
+                          n_cells=[int(v) for v in n_cells], patch_size=[int(v) for v in patch_size],
+                          cell_length=[float(v) for v in cell_length], timestep=float(timestep),
+                          precision=int(precision), deposit=int(deposit), use_graph=bool(use_graph),
+                          interpolator=interpolator, pusher=pusher, maxwell_solver=maxwell_solver)
+        self._species_args = []
+        self.steps_done = 0
         nd = 2 if geometry == "2Dcartesian" else 3
         p = SmiParams()
         self._keep = [s.encode() for s in (geometry, interpolator, pusher, maxwell_solver)]
@@ -80,6 +87,7 @@ class Simulation:
         check(self._L.smi_add_species(self.h, name.encode(), float(mass), float(charge), int(capacity),
                                       ctypes.byref(sid)))
         self.species.append(name)
+        self._species_args.append(dict(name=name, mass=float(mass), charge=float(charge), capacity=int(capacity)))
         return sid.value
 
     def set_particles(self, s, x, y, z, px, py, pz, w):
@@ -164,6 +172,41 @@ class Simulation:
     # ---- time loop
     def step(self, n=1):
         check(self._L.smi_step(self.h, int(n)))
+        self.steps_done += int(n)
+
+    # ---- checkpoint / restart (the reference dumps HDF5, Checkpoint.cpp:212-587;
+    # here a NumPy archive of the same state: every field array, every
+    # species' particles in canonical order, the step count)
+    def checkpoint(self, path):
+        """Write the full simulation state to `path` (.npz)."""
+        import json
+        data = {"meta": np.array(json.dumps(dict(args=self._args, species=self._species_args,
+                                                 steps_done=self.steps_done)))}
+        for f in FIELDS:
+            data["field_" + f] = self.get_field(f)
+        for s in range(len(self.species)):
+            p = self.get_particles(s)
+            for k in ("x", "y", "z", "px", "py", "pz", "w"):
+                data["sp%d_%s" % (s, k)] = p[k]
+        np.savez(path, **data)
+
+    @classmethod
+    def restart(cls, path, **overrides):
+        """A Simulation holding the state written by `checkpoint`
+        (constructor arguments may be overridden, e.g. precision)."""
+        import json
+        with np.load(path, allow_pickle=False) as z:
+            meta = json.loads(str(z["meta"]))
+            args = dict(meta["args"])
+            args.update(overrides)
+            sim = cls(**args)
+            for s, sp in enumerate(meta["species"]):
+                sid = sim.add_species(sp["name"], sp["mass"], sp["charge"], sp["capacity"])
+                sim.set_particles(sid, *[z["sp%d_%s" % (s, k)] for k in ("x", "y", "z", "px", "py", "pz", "w")])
+            for f in FIELDS:
+                sim.set_field(f, z["field_" + f])
+            sim.steps_done = int(meta["steps_done"])
+        return sim
 
     def synchronize(self):
         check(self._L.smi_synchronize(self.h))

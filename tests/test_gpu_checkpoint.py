@@ -1,0 +1,48 @@
+"""Checkpoint / restart: a run restarted from a checkpoint continues as the
+uninterrupted run (bit for bit in the deterministic FP64 form)."""
+import numpy as np
+import pytest
+
+import smilei_amd as sa
+from tests.plasma import cfl_dt, load, thermal
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz")
+
+
+def make(precision, deposit):
+    shape, dx = (32, 32, 32), (0.2, 0.2, 0.2)
+    sim = sa.Simulation("3Dcartesian", 2, shape, (16, 16, 16), dx, cfl_dt(dx), precision=precision, deposit=deposit)
+    for k, (name, m, q, sig) in enumerate([("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / 42.85)]):
+        p = thermal(shape, dx, 8, sig, 60 + k)
+        load(sim, sim.add_species(name, m, q, len(p["x"]) + 100), p)
+    return sim
+
+
+@pytest.mark.parametrize("precision,deposit,tol", [(sa.PLAIN_FP64, sa.DEPOSIT_DETERMINISTIC, 0.0),
+                                                   (sa.MIXED, sa.DEPOSIT_ATOMIC, 1e-6)])
+def test_restart_continues_the_run(tmp_path, precision, deposit, tol):
+    a = make(precision, deposit)
+    a.step(5)
+    path = str(tmp_path / "ckpt.npz")
+    a.checkpoint(path)
+    a.step(5)
+    b = sa.Simulation.restart(path)
+    assert b.steps_done == 5 and b.species == a.species
+    b.step(5)
+    assert b.steps_done == a.steps_done == 10
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        for k in ("x", "y", "z", "px", "py", "pz", "w"):
+            if tol == 0.0:
+                np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
+            else:
+                assert np.abs(pa[k] - pb[k]).max() <= tol * np.abs(pa[k]).max(), k
+    for f in FIELDS:
+        fa, fb = a.get_field(f), b.get_field(f)
+        if tol == 0.0:
+            np.testing.assert_array_equal(fa, fb, err_msg=f)
+        else:
+            assert np.abs(fa - fb).max() <= tol * max(np.abs(fa).max(), 1e-300), f
