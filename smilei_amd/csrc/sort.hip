@@ -98,28 +98,54 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_bsum(int32_t *__restrict_
 }
 
 // out[i] = bsum[block] + exclusive prefix inside the block; out[n] = total.
+// Each thread owns 16 consecutive entries, moved as four 16-byte vectors
+// (the entries of a full tile are 64-byte aligned); the ragged last tile
+// takes the scalar path.
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_apply(const int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bsum,
              int32_t *__restrict__ out)
 {
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    const bool full = (int64_t)(blockIdx.x + 1) * kScanTile <= n;
     int v[kScanItems];
     int s = 0;
+    if (full) {
+        const int4 *src = reinterpret_cast<const int4 *>(in + base);
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int64_t i = base + k;
-        v[k] = i < n ? in[i] : 0;
-        s += v[k];
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const int4 t = src[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0;
     }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     int tot;
     int ex = block_exclusive_scan(s, tot) + bsum[blockIdx.x];
+    int o[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const int64_t i = base + k;
-        if (i < n) out[i] = ex;
-        if (i == n - 1) out[n] = ex + v[k];
+        o[k] = ex;
         ex += v[k];
     }
+    if (full) {
+        int4 *dst = reinterpret_cast<int4 *>(out + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) dst[q] = make_int4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n) out[base + k] = o[k];
+    }
+    // the grand total after the last entry
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k == n - 1) out[n] = o[k] + v[k];
 }
 
 // Scatter of slot indices into their cell buckets.  Consecutive slots mostly
