@@ -163,9 +163,13 @@ int smi_profile_step(smi_context *ctx, float *ms);
 void *smi_stream(smi_context *ctx);
 /* Launch-shape knobs of the particle kernels, searched by smi_autotune (no
  * effect on results beyond the order of the atomic J summation).
- * SMI_TUNE_INTERLEAVE (mixed form): 1 = lane l of a wavefront takes the l-th
- * slice of its cluster's cell-sorted particles (lanes in distinct cells),
- * 0 = consecutive particles.  SMI_TUNE_THREADS: workgroup size per 4^3
+ * SMI_TUNE_INTERLEAVE (mixed form and atomic FP64 form): 1 = the sort
+ * stores each cluster's particles lane-interleaved (64 columns of
+ * consecutive cell-sorted ranks, stored row by row), so that the 64 lanes of
+ * a wavefront load consecutive slots and hold particles of distinct cells;
+ * 0 = cell-major storage (lanes share cells).  Changing it re-sorts the
+ * current state; smi_get_particles returns cell-major order either way.
+ * SMI_TUNE_THREADS: workgroup size per 4^3
  * cluster, 64 | 128 | 256 | 512 (particles per thread = cluster particles /
  * threads).  SMI_TUNE_PERSISTENT (mixed form): 0 = one workgroup per
  * cluster of the grid, 1 = a grid of the resident workgroups pulling the

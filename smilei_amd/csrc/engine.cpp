@@ -37,6 +37,7 @@ struct SpeciesState {
     void *buf[2][7] = {};          // x y z ux uy uz w: FP64, or FP32 (offsets) in the mixed form
     uint32_t *keys[2] = {};        // cell keys of the particles of buffer b
     SortBufs sb{};
+    bool lay = false;              // sb.perm holds the lane-interleaved cluster layout
     double *scratch = nullptr;     // 7*cap, deterministic mode
     SpeciesK k{};
     SpeciesKM km{};
@@ -91,7 +92,7 @@ struct smi_context {
     int *err = nullptr;            // device flag raised by kernels on corrupt particles
     int cur = 0;                   // particle buffer holding the current state
     bool mixed = false;            // SMI_MIXED form
-    int interleave = 1;            // lane-to-particle mapping of the particle kernel (tuning)
+    int interleave = 1;            // lane-interleaved cluster layout of the sort (tuning)
     int threads = 256;             // workgroup size of the particle kernel (tuning)
     int persistent = 0;            // persistent work-list form of the mixed kernel (tuning)
     int cell_order = 1;            // canonical order inside a cell (atomic deposit may drop it)
@@ -119,6 +120,31 @@ namespace {
 void check_ctx(const smi_context *c)
 {
     if (!c) throw Error(SMI_ERR_INVALID, "null context");
+}
+
+// The tile-deposit kernels (mixed form, atomic FP64 form) read each cluster
+// in the lane-interleaved layout unless the tuning turned it off; the
+// deterministic deposit walks the cells in order and keeps cell-major order.
+bool want_interleaved(const smi_context *c)
+{
+    return c->p.deposit != SMI_DEPOSIT_DETERMINISTIC && c->interleave;
+}
+
+// The particle kernel records each input particle's cell-major position
+// (the tie-break of the canonical in-cell order) when the input is stored
+// interleaved and the next sort keeps the canonical order.
+bool want_rank(const smi_context *c)
+{
+    return want_interleaved(c) && c->cell_order;
+}
+
+// Sort of species S from keys; tie: the kernel's recorded positions (the
+// step's sort) or null (the slots themselves: freshly loaded particles).
+void sort_species(smi_context *c, SpeciesState &S, const uint32_t *keys, bool counted, bool ordered,
+                  const int32_t *tie = nullptr)
+{
+    S.lay = want_interleaved(c);
+    launch_sort(c->g, keys, S.n, S.sb, c->st, counted, ordered, S.lay, tie);
 }
 
 void drop_graphs(smi_context *c)
@@ -169,6 +195,8 @@ void upload_mix(smi_context *c)
             m.perm = s.sb.perm;
             m.cell_off = s.sb.cell_off;
             m.counts = s.sb.counts;
+            m.rank = want_rank(c) ? s.sb.rank : nullptr;
+            m.lay = want_interleaved(c) ? 1 : 0;
             v.push_back(m);
         }
     SMI_HIP(hipMemcpyAsync(c->d_mix, v.data(), v.size() * sizeof(MixSpecies), hipMemcpyHostToDevice, c->st));
@@ -179,7 +207,7 @@ void upload_mix(smi_context *c)
 // any graph capture).
 void refresh_persistent(smi_context *c)
 {
-    c->persistent_blocks = c->mixed && c->persistent ? mixed_persistent_blocks(c->g, c->interleave != 0, c->threads) : 0;
+    c->persistent_blocks = c->mixed && c->persistent ? mixed_persistent_blocks(c->g, c->threads) : 0;
 }
 
 // The particle kernel launches of one step reading buffer a, writing 1 - a.
@@ -197,8 +225,8 @@ void enqueue_particles(smi_context *c, int a)
             const int n = std::min(kMaxMerged, ns - g0);
             double cr_max = 0.;
             for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
-            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->interleave != 0,
-                                   c->threads, c->xcd != 0, c->persistent ? c->persistent_blocks : 0, c->d_work,
+            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->threads, c->xcd != 0,
+                                   c->persistent ? c->persistent_blocks : 0, c->d_work,
                                    c->d_work + g.npatches * g.clpp, c->st);
         }
     } else {
@@ -225,11 +253,28 @@ void upload_fp(smi_context *c)
             f.keys = s.keys[1 - k];
             f.counts = s.sb.counts;
             f.scratch = s.scratch;
+            f.rank = want_rank(c) ? s.sb.rank : nullptr;
+            f.lay = want_interleaved(c) ? 1 : 0;
             f.cap = s.cap;
             v.push_back(f);
         }
     SMI_HIP(hipMemcpyAsync(c->d_fp, v.data(), v.size() * sizeof(FpSpecies), hipMemcpyHostToDevice, c->st));
     SMI_HIP(hipStreamSynchronize(c->st));  // v is released on return
+}
+
+// After a change of the layout or cell-order knobs: every species' current
+// order re-stored in the wanted layout (the order itself is kept), launch
+// descriptors refreshed.
+void apply_layout(smi_context *c)
+{
+    const bool want = want_interleaved(c);
+    for (auto &S : c->sp) {
+        launch_relayout(c->g, S.keys[c->cur], S.n, S.sb, S.lay, want, c->st);
+        S.lay = want;
+    }
+    upload_fp(c);
+    upload_mix(c);
+    SMI_HIP(hipStreamSynchronize(c->st));
 }
 
 // One PIC time step reading particle buffer c->cur and writing 1 - c->cur.
@@ -253,7 +298,8 @@ void enqueue_step(smi_context *c, bool timed)
     launch_maxwell(g, f, c->st);
     if (timed) SMI_HIP(hipEventRecord(c->ev[3], c->st));
     const bool ordered = det || c->cell_order;
-    for (auto &s : c->sp) launch_sort(g, s.keys[1 - a], s.n, s.sb, c->st, /*counted=*/true, ordered);
+    for (auto &s : c->sp)
+        sort_species(c, s, s.keys[1 - a], /*counted=*/true, ordered, want_rank(c) ? s.sb.rank : nullptr);
     if (timed) SMI_HIP(hipEventRecord(c->ev[4], c->st));
 }
 
@@ -403,6 +449,8 @@ int smi_destroy(smi_context *c)
         (void)hipFree(s.sb.counts);
         (void)hipFree(s.sb.cell_off);
         (void)hipFree(s.sb.perm);
+        (void)hipFree(s.sb.tmp);
+        (void)hipFree(s.sb.rank);
         (void)hipFree(s.sb.bsum);
         (void)hipFree(s.scratch);
     }
@@ -440,6 +488,10 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.sb.counts = dalloc<int32_t>(c->g.ncells);
     s.sb.cell_off = dalloc<int32_t>(c->g.ncells + 1);
     s.sb.perm = dalloc<int32_t>(capacity);
+    if (c->p.deposit != SMI_DEPOSIT_DETERMINISTIC) {
+        s.sb.tmp = dalloc<int32_t>(capacity);
+        s.sb.rank = dalloc<int32_t>(capacity);
+    }
     s.sb.nbsum = sort_bsum_size(c->g.ncells);
     s.sb.bsum = dalloc<int32_t>(s.sb.nbsum);
     if (c->p.deposit == SMI_DEPOSIT_DETERMINISTIC) s.scratch = dalloc<double>(7 * capacity);
@@ -529,7 +581,7 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
         }
         launch_bin_initial(c->g, S.view(c->cur), n, S.keys[c->cur], c->st);
     }
-    launch_sort(c->g, S.keys[c->cur], n, S.sb, c->st);
+    sort_species(c, S, S.keys[c->cur], false, true);
     SMI_HIP(hipStreamSynchronize(c->st));
     return SMI_OK;
     SMI_CATCH
@@ -612,7 +664,7 @@ void init_particles(smi_context *c, int s, int64_t n, int ppc, const int32_t *d_
                 c->st);
     S.n = n;
     if (!c->mixed) launch_bin_initial(g, S.view(c->cur), n, S.keys[c->cur], c->st);
-    launch_sort(g, S.keys[c->cur], n, S.sb, c->st);
+    sort_species(c, S, S.keys[c->cur], false, true);
     SMI_HIP(hipStreamSynchronize(c->st));
 }
 
@@ -715,7 +767,7 @@ int smi_init_species_like(smi_context *c, int s, int src, double weight_scale, d
                      weight_scale, 0.0, mj.n ? mj.buf.p : nullptr, mj.n, drift_ux,
                      seed * 0x100000001B3ull + (uint64_t)s, c->st);
     S.n = R.n;
-    launch_sort(c->g, S.keys[c->cur], S.n, S.sb, c->st);
+    sort_species(c, S, S.keys[c->cur], false, true);
     SMI_HIP(hipStreamSynchronize(c->st));
     return SMI_OK;
     SMI_CATCH
@@ -828,12 +880,22 @@ int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, do
         SMI_HIP(hipMemcpy(perm.data(), S.sb.perm, n * 4, hipMemcpyDeviceToHost));
         SMI_HIP(hipMemcpy(keys.data(), S.keys[c->cur], n * 4, hipMemcpyDeviceToHost));
     }
+    if (S.lay && n) {
+        // back to cell-major order: rank p of a cluster sits at slot cluster_slot(p)
+        std::vector<int32_t> off(c->g.ncells + 1), lay(perm);
+        SMI_HIP(hipMemcpy(off.data(), S.sb.cell_off, off.size() * 4, hipMemcpyDeviceToHost));
+        const int cpc = c->g.cells_per_cl;
+        for (int64_t cb = 0; cb < c->g.ncells; cb += cpc) {
+            const int a = off[cb], m = off[cb + cpc] - a;
+            for (int p = 0; p < m; ++p) perm[a + p] = lay[a + cluster_slot(p, m)];
+        }
+    }
     double *dst[7] = {x, y, z, px, py, pz, w};
     if (c->mixed) {
         // FP64 global positions in sorted order, built on the device
         std::vector<double *> t(7, nullptr);
         for (auto &q : t) q = dalloc<double>(n);
-        launch_from_mixed(c->g, S.mview(c->cur), S.keys[c->cur], S.sb.perm, n,
+        launch_from_mixed(c->g, S.mview(c->cur), S.keys[c->cur], S.sb.perm, S.sb.cell_off, S.lay, n,
                           PartPtrs{t[0], t[1], t[2], t[3], t[4], t[5], t[6]}, c->st);
         SMI_HIP(hipStreamSynchronize(c->st));
         for (int q = 0; q < 7; ++q)
@@ -987,6 +1049,8 @@ int smi_set_tuning(smi_context *c, int key, int value)
     }
     drop_graphs(c);
     refresh_persistent(c);
+    SMI_HIP(hipSetDevice(c->p.device));
+    apply_layout(c);
     return SMI_OK;
     SMI_CATCH
 }
@@ -1014,23 +1078,29 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     SMI_HIP(hipSetDevice(c->p.device));
     SMI_HIP(hipStreamSynchronize(c->st));
     drop_graphs(c);
-    // candidates: lane mapping (mixed form) x workgroup size; each timed on
-    // the live particle state with the particle launches of one step.  The
-    // trial launches only write the output ping-pong buffer (rewritten by the
-    // next step), the destination histograms (cleared below) and, in the
-    // atomic deposit, J (cleared at the start of every step).
+    // candidates: cluster layout (tile-deposit forms) x persistent form
+    // (mixed) x cluster order x workgroup size; each timed on the live
+    // particle state with the particle launches of one step.  The trial
+    // launches only write the output ping-pong buffer (rewritten by the next
+    // step), the destination histograms (cleared below) and, in the atomic
+    // deposit, J (cleared at the start of every step).  A layout is tried
+    // after re-sorting the current state into it.
     const int maps[2] = {1, 0};
-    const int nmap = c->mixed ? 2 : 1, npers = c->mixed ? 2 : 1;
+    const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
+    const int nmap = det ? 1 : 2, npers = c->mixed ? 2 : 1;
     const int sizes[3] = {128, 256, 512};
     int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent, best_xcd = c->xcd;
     float best = 3.4e38f;
+    for (int m = 0; m < nmap; ++m) {
+    if (!det && c->interleave != maps[m]) {
+        c->interleave = maps[m];
+        apply_layout(c);
+    }
     for (int pm = 0; pm < npers; ++pm)
     for (int xo = 0; xo < 2; ++xo)
-    for (int m = 0; m < nmap; ++m)
         for (int z = 0; z < 3; ++z) {
             if (sizes[z] > particle_max_threads(c->g, c->mixed)) continue;
             if (pm && xo) continue;  // the persistent form takes clusters from its work list
-            c->interleave = c->mixed ? maps[m] : c->interleave;
             c->threads = sizes[z];
             c->persistent = pm;
             c->xcd = xo;
@@ -1051,11 +1121,13 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
                 best_xcd = c->xcd;
             }
         }
+    }
     c->interleave = best_map;
     c->threads = best_threads;
     c->persistent = best_pers;
     c->xcd = best_xcd;
     refresh_persistent(c);
+    apply_layout(c);
     for (auto &s : c->sp) launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
     SMI_HIP(hipStreamSynchronize(c->st));
     if (best_ms) *best_ms = best;

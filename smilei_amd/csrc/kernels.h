@@ -45,6 +45,8 @@ struct FpSpecies {
     uint32_t *keys;                  // destination keys written for B
     int32_t *counts;                 // destination histogram
     double *scratch;                 // 7*cap doubles (deterministic deposit)
+    int32_t *rank;                   // or null: cell-major input position of each output slot
+    int lay;                         // the input's clusters are stored lane-interleaved
     int64_t cap;
 };
 // One block per 4-cell cluster of every patch runs the particles of the ns
@@ -79,10 +81,42 @@ void launch_ghost_sum_and_fold(const Geom &g, double *pj, const JPtrs &J, hipStr
 int64_t patch_j_size(const Geom &g);  // doubles per patch (Jx+Jy+Jz)
 
 // ---- sort (sort.hip) --------------------------------------------------
+// Lane-interleaved storage of a cluster.  The n particles of a cluster, in
+// cell-major order (rank p = 0..n-1), are dealt into 64 columns of
+// consecutive ranks (the first `full` columns hold q = ceil(n/64) ranks, the
+// others q-1) and stored row by row: slot = row*64 + column.  A wavefront
+// reading 64 consecutive slots then loads coalesced, and its lanes hold
+// particles about n/64 ranks apart, i.e. in different cells: their LDS
+// stencil accesses spread over the tile instead of piling onto the same
+// addresses.  cluster_rank is the inverse map.
+__host__ __device__ inline int cluster_slot(int p, int n)
+{
+    const int q = (n + 63) >> 6, full = n - 64 * (q - 1);
+    int col, row;
+    if (p < full * q) {
+        col = p / q;
+        row = p - col * q;
+    } else {
+        const int r = p - full * q;
+        const int k = r / (q - 1);
+        col = full + k;
+        row = r - k * (q - 1);
+    }
+    return row * 64 + col;
+}
+__host__ __device__ inline int cluster_rank(int m, int n)
+{
+    const int q = (n + 63) >> 6, full = n - 64 * (q - 1);
+    const int row = m >> 6, col = m & 63;
+    return col < full ? col * q + row : full * q + (col - full) * (q - 1) + row;
+}
+
 struct SortBufs {
     int32_t *counts;   // ncells, kept at zero between calls
     int32_t *cell_off; // ncells + 1
     int32_t *perm;     // cap
+    int32_t *tmp;      // cap (cell-major order before the interleaved layout)
+    int32_t *rank;     // cap (tie-break of the canonical order: input position, cell-major)
     int32_t *bsum;     // scan scratch
     int64_t nbsum;
 };
@@ -92,8 +126,19 @@ int64_t sort_bsum_size(int64_t ncells);
 // the atomic scatter): fills cell_off and perm from keys[0..n).  With
 // counted = true the histogram of destination cells is already in sb.counts
 // (the particle kernel builds it); otherwise it is computed from the keys.
+// interleaved: each cluster's range of perm is stored in the lane-
+// interleaved layout (cluster_slot) instead of cell-major order; cell_off
+// then still gives every cluster's range, the cells' sub-ranges refer to
+// cell-major ranks.  The canonical order breaks ties by the source slot, or
+// by tie[slot] when given (the slot's cell-major position in the previous
+// sorted order, for sources stored interleaved).
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st,
-                 bool counted = false, bool in_cell_order = true);
+                 bool counted = false, bool in_cell_order = true, bool interleaved = false,
+                 const int32_t *tie = nullptr);
+// Re-stores the current sorted order in another cluster layout (perm via
+// tmp); keys are those the order was sorted from.
+void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, bool from_interleaved,
+                     bool to_interleaved, hipStream_t st);
 // Keys of freshly loaded particles (positions in A, slot order = input).
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
 
@@ -108,6 +153,8 @@ struct MixSpecies {
     uint32_t *keysB;                // keys written for the output buffer
     const int32_t *perm, *cell_off;  // sorted view of the input
     int32_t *counts;                // destination histogram
+    int32_t *rank;                  // or null: cell-major input position of each output slot
+    int lay;                        // the input's clusters are stored lane-interleaved
 };
 // Particle kernel of the mixed form over ns <= kMaxMerged species that share
 // one J tile per cluster; cr_max = the largest current contribution of those
@@ -115,13 +162,15 @@ struct MixSpecies {
 // selects the persistent form: that many workgroups pull the non-empty
 // clusters from a work list (work: one int per cluster, wcount: 2 ints).
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, bool interleave, int threads, bool xcd,
-                            int persistent_blocks, int *work, int *wcount, hipStream_t st);
+                            const JPtrs &J, int *err, int threads, bool xcd, int persistent_blocks, int *work,
+                            int *wcount, hipStream_t st);
 // Resident workgroups of the persistent mixed kernel on the current device.
-int mixed_persistent_blocks(const Geom &g, bool interleave, int threads);
+int mixed_persistent_blocks(const Geom &g, int threads);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
-void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm, int64_t n,
-                       const PartPtrs &X, hipStream_t st);
+// FP64 particles in cell-major sorted order (cell_off/interleaved: the
+// layout of perm, see launch_sort).
+void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,
+                       const int32_t *cell_off, bool interleaved, int64_t n, const PartPtrs &X, hipStream_t st);
 void launch_ukin_mixed(const MixPtrs &M, const int32_t *perm, int64_t n, double *partials, int nblocks,
                        hipStream_t st);
 void launch_rho_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,

@@ -134,17 +134,20 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
     const int32_t *__restrict__ perm = Sp.perm;
     uint32_t *__restrict__ keys = Sp.keys;
     int32_t *__restrict__ counts = Sp.counts;
+    int32_t *__restrict__ rank = Sp.rank;
+    const int lay = Sp.lay;
     double *__restrict__ scratch = Sp.scratch;
     const int64_t cap = Sp.cap;
     const int s = Sp.cell_off[cl.gbase], e = Sp.cell_off[cl.gbase + g.cells_per_cl];
     const int N = e - s;
-    const int q = DEPOSIT ? (N + 63) >> 6 : 1;
-    const int span = DEPOSIT ? 64 * q : N;
-    for (int kk = threadIdx.x; kk < span; kk += blockDim.x) {
-        const int rel = DEPOSIT ? (kk & 63) * q + (kk >> 6) : kk;
-        if (rel >= N) continue;
+    // consecutive slots per wavefront (lanes in different cells when the
+    // sort stored the cluster lane-interleaved)
+    for (int rel = threadIdx.x; rel < N; rel += blockDim.x) {
         const int jslot = s + rel;
         const int i = perm[jslot];
+        // cell-major position of this input particle: the tie-break of the
+        // canonical in-cell order at the next sort
+        if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
         double x[3] = {A.x[i], A.y[i], ND == 3 ? A.z[i] : 0.};
         double u[3] = {A.px[i], A.py[i], A.pz[i]};
         const double w = A.w[i];

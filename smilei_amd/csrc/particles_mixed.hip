@@ -280,8 +280,11 @@ __device__ inline uint32_t global_key3(const Geom &g, const int *c, int *pc)
 // One workgroup runs the particles of every species of the launch
 // (specs[0..ns)) of the 4^3 cluster `cluster` against one field tile and
 // one J tile.  Uniform across the workgroup; returns at once for an empty
-// cluster.
-template <int ORDER, bool INTERLEAVE>
+// cluster.  Thread t takes the slots t, t + blockDim.x, ... of the
+// cluster's range of perm: a wavefront reads 64 consecutive slots, whose
+// particles lie in different cells when the sort stored the range in the
+// lane-interleaved layout (cluster_slot).
+template <int ORDER>
 __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict__ specs, int ns, double cr_max,
                                      const GatherFields &F, const JPtrs &J, int *__restrict__ err, int cluster)
 {
@@ -301,7 +304,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     // wavefronts of the main loop run the narrow stencil with all lanes;
     // entry = old and new deltas, charge weight in quanta, packed stencil
     // origin and shifts
-    constexpr int QCAP = ORDER == 2 ? 256 : 128;
+    constexpr int QCAP = ORDER == 2 ? 224 : 128;
     __shared__ float sQ[QCAP][8];
     __shared__ int sQn;
 
@@ -336,15 +339,16 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     uint32_t *__restrict__ keysB = S.keysB;
     const int32_t *__restrict__ perm = S.perm;
     int32_t *__restrict__ counts = S.counts;
+    int32_t *__restrict__ rank = S.rank;
+    const int lay = S.lay;
     const int s = S.cell_off[cl.gbase], e = S.cell_off[cl.gbase + 64];
     const int N = e - s;
-    const int q = INTERLEAVE ? (N + 63) >> 6 : 1;
-    const int span = INTERLEAVE ? 64 * q : N;
-    for (int kk = threadIdx.x; kk < span; kk += blockDim.x) {
-        const int rel = INTERLEAVE ? (kk & 63) * q + (kk >> 6) : kk;
-        if (rel >= N) continue;
+    for (int rel = threadIdx.x; rel < N; rel += blockDim.x) {
         const int jslot = s + rel;
         const int i = perm[jslot];
+        // cell-major position of this input particle: the tie-break of the
+        // canonical in-cell order at the next sort
+        if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
         const int lc = (int)((int64_t)(keysA[i] & kMask) - cl.gbase);
         float o[3] = {A.ox[i], A.oy[i], A.oz[i]};
         float u[3] = {A.ux[i], A.uy[i], A.uz[i]};
@@ -527,19 +531,21 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 }
 
 // One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
-template <int ORDER, bool INTERLEAVE>
-__global__ void __launch_bounds__(512, ORDER == 2 ? 3 : 2)
+// Order 2: at most 128 VGPRs and 40,288 B of LDS, so that four wavefronts
+// per SIMD are resident (four 256-thread workgroups per CU).
+template <int ORDER>
+__global__ void __launch_bounds__(512, ORDER == 2 ? 4 : 2)
 k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
                   int *__restrict__ err, int xcd)
 {
     const int b = xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x;
-    mixed_cluster<ORDER, INTERLEAVE>(g, specs, ns, cr_max, F, J, err, b);
+    mixed_cluster<ORDER>(g, specs, ns, cr_max, F, J, err, b);
 }
 
 // Persistent form: a grid sized to the resident workgroups pulls clusters
 // from the work list (the non-empty clusters, wcount[0] of them) through the
 // queue head wcount[1]; every workgroup leaves once the list is exhausted.
-template <int ORDER, bool INTERLEAVE>
+template <int ORDER>
 __global__ void __launch_bounds__(512, ORDER == 2 ? 3 : 2)
 k_particles_mixed_list(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
                        int *__restrict__ err, const int *__restrict__ work, int *__restrict__ wcount)
@@ -552,7 +558,7 @@ k_particles_mixed_list(Geom g, const MixSpecies *__restrict__ specs, int ns, dou
         __syncthreads();
         const int item = sItem;
         if (item >= nwork) break;
-        mixed_cluster<ORDER, INTERLEAVE>(g, specs, ns, cr_max, F, J, err, work[item]);
+        mixed_cluster<ORDER>(g, specs, ns, cr_max, F, J, err, work[item]);
     }
 }
 
@@ -605,14 +611,22 @@ __global__ void k_to_mixed(Geom g, PartPtrs X, int64_t n, MixPtrs M, uint32_t *_
     M.w[i] = (float)X.w[i];
 }
 
-// Sorted view -> FP64 global positions (x = (cell + offset) * dx).
+// Sorted view -> FP64 global positions (x = (cell + offset) * dx), written
+// in cell-major order (slot j of an interleaved cluster range goes to the
+// cluster's rank cluster_rank(j)).
 __global__ void k_from_mixed(Geom g, MixPtrs M, const uint32_t *__restrict__ keys, const int32_t *__restrict__ perm,
-                             int64_t n, PartPtrs X)
+                             const int32_t *__restrict__ cell_off, int interleaved, int64_t n, PartPtrs X)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int i = perm[j];
+    const int64_t js = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (js >= n) return;
+    const int i = perm[js];
     const uint32_t gk = keys[i] & kMask;
+    int64_t j = js;
+    if (interleaved) {
+        const uint32_t cb = gk & ~63u;
+        const int s = cell_off[cb];
+        j = s + cluster_rank((int)(js - s), cell_off[cb + 64] - s);
+    }
     const int p = (int)(gk / (uint32_t)g.cpp), key = (int)(gk % (uint32_t)g.cpp);
     const int pc[3] = {p / (g.npatch[1] * g.npatch[2]), (p / g.npatch[2]) % g.npatch[1], p % g.npatch[2]};
     const int cl = key >> 6, lc = key & 63;
@@ -694,8 +708,8 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }  // namespace
 
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, bool interleave, int threads, bool xcd,
-                            int persistent_blocks, int *work, int *wcount, hipStream_t st)
+                            const JPtrs &J, int *err, int threads, bool xcd, int persistent_blocks, int *work,
+                            int *wcount, hipStream_t st)
 {
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
@@ -704,15 +718,10 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
     const int nclusters = g.npatches * g.clpp;
     const dim3 block(threads);
     if (persistent_blocks <= 0) {
-#define SMI_PM(O, I) hipLaunchKernelGGL((k_particles_mixed<O, I>), dim3(nclusters), block, 0, st, g, specs, ns, \
-                                        cr_max, F, J, err, xcd ? 1 : 0)
-        if (g.order == 2) {
-            if (interleave) SMI_PM(2, true);
-            else SMI_PM(2, false);
-        } else {
-            if (interleave) SMI_PM(4, true);
-            else SMI_PM(4, false);
-        }
+#define SMI_PM(O) hipLaunchKernelGGL((k_particles_mixed<O>), dim3(nclusters), block, 0, st, g, specs, ns, cr_max, \
+                                     F, J, err, xcd ? 1 : 0)
+        if (g.order == 2) SMI_PM(2);
+        else SMI_PM(4);
 #undef SMI_PM
         SMI_LAUNCH_CHECK();
         return;
@@ -722,29 +731,20 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
     hipLaunchKernelGGL(k_worklist, dim3((nclusters + 255) / 256), dim3(256), 0, st, g, specs, ns, work, wcount);
     SMI_LAUNCH_CHECK();
     const int nblocks = std::min(nclusters, persistent_blocks);
-#define SMI_PL(O, I) hipLaunchKernelGGL((k_particles_mixed_list<O, I>), dim3(nblocks), block, 0, st, g, specs, ns, \
-                                        cr_max, F, J, err, (const int *)work, wcount)
-    if (g.order == 2) {
-        if (interleave) SMI_PL(2, true);
-        else SMI_PL(2, false);
-    } else {
-        if (interleave) SMI_PL(4, true);
-        else SMI_PL(4, false);
-    }
+#define SMI_PL(O) hipLaunchKernelGGL((k_particles_mixed_list<O>), dim3(nblocks), block, 0, st, g, specs, ns, \
+                                     cr_max, F, J, err, (const int *)work, wcount)
+    if (g.order == 2) SMI_PL(2);
+    else SMI_PL(4);
 #undef SMI_PL
     SMI_LAUNCH_CHECK();
 }
 
-int mixed_persistent_blocks(const Geom &g, bool interleave, int threads)
+int mixed_persistent_blocks(const Geom &g, int threads)
 {
     int dev = 0, cus = 0, per_cu = 0;
     SMI_HIP(hipGetDevice(&dev));
     SMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const void *fn = nullptr;
-    if (g.order == 2)
-        fn = interleave ? (const void *)k_particles_mixed_list<2, true> : (const void *)k_particles_mixed_list<2, false>;
-    else
-        fn = interleave ? (const void *)k_particles_mixed_list<4, true> : (const void *)k_particles_mixed_list<4, false>;
+    const void *fn = g.order == 2 ? (const void *)k_particles_mixed_list<2> : (const void *)k_particles_mixed_list<4>;
     SMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
     return std::max(1, cus * std::max(per_cu, 1));
 }
@@ -756,11 +756,12 @@ void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs 
     SMI_LAUNCH_CHECK();
 }
 
-void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm, int64_t n,
-                       const PartPtrs &X, hipStream_t st)
+void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,
+                       const int32_t *cell_off, bool interleaved, int64_t n, const PartPtrs &X, hipStream_t st)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_from_mixed, dim3(nblk(n, 256)), dim3(256), 0, st, g, M, keys, perm, n, X);
+    hipLaunchKernelGGL(k_from_mixed, dim3(nblk(n, 256)), dim3(256), 0, st, g, M, keys, perm, cell_off,
+                       interleaved ? 1 : 0, n, X);
     SMI_LAUNCH_CHECK();
 }
 

@@ -151,9 +151,11 @@ k_scan_apply(const int32_t *__restrict__ in, int64_t n, const int32_t *__restric
 // Scatter of slot indices into their cell buckets.  Consecutive slots mostly
 // keep their cell, so lanes of a wavefront with equal keys form runs: the
 // run head takes `run` places with one atomic and the lanes fill them in
-// slot order.  The count-down returns the histogram to zero.
+// slot order.  The count-down returns the histogram to zero.  cpc > 0 (the
+// cells per cluster): the cell-major place is stored at its lane-
+// interleaved slot of the cluster (cluster_slot).
 __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
-                          int32_t *__restrict__ counts, int32_t *__restrict__ perm)
+                          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -171,58 +173,103 @@ __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const in
     if (g == 0xFFFFFFFFu) return;
     // the run occupies places [c - run, c) counted from the bucket start
     const int run = run_next - my_head;
-    perm[cell_off[g] + ch - run + (lane - my_head)] = (int32_t)i;
+    const int pos = cell_off[g] + ch - run + (lane - my_head);
+    if (cpc) {
+        const uint32_t cb = g & ~(uint32_t)(cpc - 1);
+        const int s = cell_off[cb];
+        perm[s + cluster_slot(pos - s, cell_off[cb + cpc] - s)] = (int32_t)i;
+    } else {
+        perm[pos] = (int32_t)i;
+    }
 }
 
 // Canonical order inside a cell: by direction of motion, then source slot.
 // One thread per cell.  Buckets of up to kFixRegs particles are ranked in
 // registers (all loads issued together, rank = number of smaller
 // (direction, slot) pairs; the pairs are distinct because slots are);
-// larger buckets fall back to an insertion sort in memory.
+// larger buckets fall back to an insertion sort in memory.  The cell-major
+// input is src; cpc == 0: sorted in place (dst == src); cpc > 0: every
+// particle is written to its lane-interleaved slot of the cluster in dst.
+// The tie-break value of source slot v is tie[v] when tie is given, else v.
 constexpr int kFixRegs = 16;
 
 __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
-                             int32_t *__restrict__ perm)
+                             int32_t *src, int32_t *dst, int cpc, const int32_t *__restrict__ tie)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncells) return;
     const int a = cell_off[g], b = cell_off[g + 1];
     const int n = b - a;
-    if (n < 2) return;
+    int cs = 0, cn = 0;
+    if (cpc) {
+        const int64_t cb = g & ~(int64_t)(cpc - 1);
+        cs = cell_off[cb];
+        cn = cell_off[cb + cpc] - cs;
+    }
+    // final place of the particle of cell-major position pos
+    auto put = [&](int pos, int32_t v) { dst[cpc ? cs + cluster_slot(pos - cs, cn) : pos] = v; };
+    if (n < 2) {
+        if (cpc && n == 1) put(a, src[a]);
+        return;
+    }
     if (n <= kFixRegs) {
         int32_t v[kFixRegs];
         uint64_t k[kFixRegs];
 #pragma unroll
-        for (int i = 0; i < kFixRegs; ++i) v[i] = i < n ? perm[a + i] : 0;
+        for (int i = 0; i < kFixRegs; ++i) v[i] = i < n ? src[a + i] : 0;
 #pragma unroll
         for (int i = 0; i < kFixRegs; ++i)
-            k[i] = i < n ? ((uint64_t)(keys[v[i]] >> 27) << 32) | (uint32_t)v[i] : ~0ull;
+            k[i] = i < n ? ((uint64_t)(keys[v[i]] >> 27) << 32) | (uint32_t)(tie ? tie[v[i]] : v[i]) : ~0ull;
         bool sorted = true;
 #pragma unroll
         for (int i = 1; i < kFixRegs; ++i) sorted &= k[i - 1] < k[i] || i >= n;
-        if (sorted) return;
+        if (sorted) {
+            if (cpc)
+#pragma unroll
+                for (int i = 0; i < kFixRegs; ++i)
+                    if (i < n) put(a + i, v[i]);
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < kFixRegs; ++i) {
             int r = 0;
 #pragma unroll
             for (int j = 0; j < kFixRegs; ++j) r += k[j] < k[i] ? 1 : 0;
-            if (i < n) perm[a + r] = v[i];
+            if (i < n) put(a + r, v[i]);
         }
         return;
     }
+    int32_t *perm = src;
     for (int i = a + 1; i < b; ++i) {
         const int32_t v = perm[i];
-        const uint64_t kv = ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)v;
+        const uint64_t kv = ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)(tie ? tie[v] : v);
         int j = i - 1;
         while (j >= a) {
             const int32_t u = perm[j];
-            const uint64_t ku = ((uint64_t)(keys[u] >> 27) << 32) | (uint32_t)u;
+            const uint64_t ku = ((uint64_t)(keys[u] >> 27) << 32) | (uint32_t)(tie ? tie[u] : u);
             if (ku <= kv) break;
             perm[j + 1] = u;
             --j;
         }
         perm[j + 1] = v;
     }
+    if (cpc)
+        for (int i = a; i < b; ++i) put(i, perm[i]);
+}
+
+// Slot j of the current order, stored with layout `from`, moved to its slot
+// under layout `to` (0: cell-major, else the cells per cluster of the
+// interleaved layout); the cluster of the particle comes from its key.
+__global__ void k_relayout(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
+                           const int32_t *__restrict__ src, int32_t *__restrict__ dst, int cpc, int from, int to)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int32_t v = src[j];
+    const uint32_t cb = (keys[v] & kKeyMask) & ~(uint32_t)(cpc - 1);
+    const int s = cell_off[cb], m = cell_off[cb + cpc] - s;
+    const int p = from ? cluster_rank((int)(j - s), m) : (int)(j - s);
+    dst[s + (to ? cluster_slot(p, m) : p)] = v;
 }
 
 // Patch and cell key of loaded particles; all get direction 13 ("stay") so
@@ -262,7 +309,7 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 int64_t sort_bsum_size(int64_t ncells) { return (ncells + kScanTile - 1) / kScanTile + 1; }
 
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st, bool counted,
-                 bool in_cell_order)
+                 bool in_cell_order, bool interleaved, const int32_t *tie)
 {
     const int64_t nc = g.ncells;
     if (n > 0 && !counted) {
@@ -279,15 +326,31 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
                        (const int32_t *)sb.bsum, sb.cell_off);
     SMI_LAUNCH_CHECK();
     if (n > 0) {
+        // cell-major places go straight to perm unless the canonical order
+        // inside the cells and the interleaved layout are both asked for:
+        // then they are ordered in tmp and laid out into perm
+        const int cpc = interleaved ? g.cells_per_cl : 0;
+        int32_t *first = in_cell_order && interleaved ? sb.tmp : sb.perm;
         hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
-                           sb.counts, sb.perm);
+                           sb.counts, first, in_cell_order ? 0 : cpc);
         SMI_LAUNCH_CHECK();
         if (in_cell_order) {
             hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                               (const int32_t *)sb.cell_off, sb.perm);
+                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie);
             SMI_LAUNCH_CHECK();
         }
     }
+}
+
+void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, bool from_interleaved,
+                     bool to_interleaved, hipStream_t st)
+{
+    if (n == 0 || from_interleaved == to_interleaved) return;
+    hipLaunchKernelGGL(k_relayout, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
+                       (const int32_t *)sb.perm, sb.tmp, g.cells_per_cl, from_interleaved ? 1 : 0,
+                       to_interleaved ? 1 : 0);
+    SMI_LAUNCH_CHECK();
+    SMI_HIP(hipMemcpyAsync(sb.perm, sb.tmp, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
 }
 
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st)
