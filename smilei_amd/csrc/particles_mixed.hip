@@ -206,14 +206,39 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
     }
 }
 
+// LDS tiles of one cluster of the mixed kernel.  Field tile: GT^3 points
+// per field from G0-GO, strides GSX, GSY, 1 (floats).  J tile: JT^3 words
+// per component from G0-JO (stencil rows [c0-H-1, c0+4+H]), strides JSX,
+// JSY, 1 and JSC between components (64-bit words).  The x-stride of 76
+// floats (not 8*8 = 64, a multiple of the LDS bank count) spreads the four
+// x-planes of cells that the lanes of a wavefront cover over different
+// banks (measured on the order-2 kernel: x-strides of 68, 76, 84 equal,
+// 72 and 80 6 % slower).  The node-changer queue takes the LDS left in the budget that
+// keeps four workgroups per CU (order 2).
+template <int ORDER>
+struct MixTile {
+    static constexpr int H = ORDER / 2;
+    static constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
+    static constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GT * GSX;
+    // order 2: J rows padded to 10 words (a y-stride of 9 = JT, x-stride 81
+    // collided more often in the bank of the 64-bit atomics: 3 % slower)
+    static constexpr int JSY = ORDER == 2 ? JT + 1 : JT, JSX = JT * JSY, JSC = JT * JSX;
+    static constexpr int kBudget = ORDER == 2 ? 40960 - 64 : 65536;
+    static constexpr int kFixed = 6 * GF * 4 + 3 * JSC * 8 + kMaxMerged * 64 * 4 + 16;
+    static constexpr int QMAX = ORDER == 2 ? 256 : 128;
+    static constexpr int QFIT = (kBudget - kFixed) / 32 / 32 * 32;
+    static constexpr int QCAP = QFIT < QMAX ? QFIT : QMAX;
+    static_assert(QCAP >= 64, "LDS budget leaves no node-changer queue");
+};
+
 // Esirkepov deposit of one particle from its old and new in-cell deltas
 // (dold, dnew in [-1/2, 1/2)), node shifts sh and the tile coordinates o3 of
 // its stencil origin; NT/NA as esirkepov3_fixed.
-template <int ORDER, int NT, int NA, int JT>
+template <int ORDER, int NT, int NA>
 __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, const float *dnew, const int *sh,
                                      const int *o3, const double *crq)
 {
-    constexpr int JT3 = JT * JT * JT;
+    using TL = MixTile<ORDER>;
     double S0[3][ORDER + 2], DS[3][ORDER + 2];
     int w0[3];
     if constexpr (NT == ORDER + 1) {
@@ -236,9 +261,9 @@ __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, 
 #pragma unroll
         for (int d = 0; d < 3; ++d) windowd<ORDER>((double)dold[d], (double)dnew[d], sh[d], S0[d], DS[d], w0[d]);
     }
-    unsigned long long *sJb = sJ + (o3[0] * JT + o3[1]) * JT + o3[2];
+    unsigned long long *sJb = sJ + o3[0] * TL::JSX + o3[1] * TL::JSY + o3[2];
     esirkepov3_fixed<ORDER, NT, NA>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
-        return &sJb[c * JT3 + (a * JT + b) * JT + k];
+        return &sJb[c * TL::JSC + a * TL::JSX + b * TL::JSY + k];
     });
 }
 
@@ -288,23 +313,19 @@ template <int ORDER>
 __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict__ specs, int ns, double cr_max,
                                      const GatherFields &F, const JPtrs &J, int *__restrict__ err, int cluster)
 {
-    constexpr int H = ORDER / 2;
-    // J tile: stencil rows [c0-H-1, c0+4+H]
-    constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
+    using TL = MixTile<ORDER>;
+    constexpr int H = TL::H, GO = TL::GO, GT = TL::GT, JO = TL::JO, JT = TL::JT;
+    constexpr int GSX = TL::GSX, GSY = TL::GSY, GF = TL::GF;
     constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
-    // field tile strides: an x-stride of 76 floats (not 8*8 = 64, a multiple
-    // of the LDS bank count) spreads the four x-planes of cells that the
-    // lanes of a wavefront cover over different banks
-    constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GT * GSX;
     __shared__ float sF[6 * GF];
-    __shared__ unsigned long long sJ[3 * JT3];
+    __shared__ unsigned long long sJ[3 * TL::JSC];
     __shared__ int sHist[kMaxMerged][64];
     // particles whose node changes on some axis: their deposit (the wider
     // stencil) is deferred to a second pass over this queue, so that the
     // wavefronts of the main loop run the narrow stencil with all lanes;
     // entry = old and new deltas, charge weight in quanta, packed stencil
     // origin and shifts
-    constexpr int QCAP = ORDER == 2 ? 224 : 128;
+    constexpr int QCAP = TL::QCAP;
     __shared__ float sQ[QCAP][8];
     __shared__ int sQn;
 
@@ -321,7 +342,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 #pragma unroll
         for (int f = 0; f < 6; ++f) sF[f * GF + a * GSX + b * GSY + c] = (float)src[f][idx];
     }
-    for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0ull;
+    for (int t = threadIdx.x; t < 3 * TL::JSC; t += blockDim.x) sJ[t] = 0ull;
     for (int t = threadIdx.x; t < kMaxMerged * 64; t += blockDim.x) sHist[t / 64][t % 64] = 0;
     if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
@@ -450,7 +471,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             // contributions in units of the cluster's fixed-point quantum
             const double cwq = (double)(sp.cw_scale * w) * inv_quantum;
             const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
-            if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER, JT>(sJ, dold, dnew, sh, o3, crq);
+            if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER>(sJ, dold, dnew, sh, o3, crq);
             // node changers: one queue slot range per wavefront
             const uint64_t gm = __ballot(moved != 0);
             if (gm) {
@@ -473,7 +494,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
                         sQ[pos][6] = (float)(sp.cw_scale * w);
                         sQ[pos][7] = __int_as_float(packed);
                     } else {
-                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JT>(sJ, dold, dnew, sh, o3, crq);
+                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1>(sJ, dold, dnew, sh, o3, crq);
                     }
                 }
             }
@@ -514,15 +535,15 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const int packed = __float_as_int(sQ[t][7]);
         const int o3[3] = {packed & 7, (packed >> 3) & 7, (packed >> 6) & 7};
         const int sh[3] = {((packed >> 9) & 3) - 1, ((packed >> 11) & 3) - 1, ((packed >> 13) & 3) - 1};
-        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JT>(sJ, dold, dnew, sh, o3, crq);
+        deposit_fixed<ORDER, ORDER + 2, ORDER + 1>(sJ, dold, dnew, sh, o3, crq);
     }
     __syncthreads();
     double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
     for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
-        const double v = fixed_decode(sJ[t], quantum);
-        if (v == 0.) continue;
         const int c = t / JT3, r = t % JT3;
         const int a = r / (JT * JT), b = (r / JT) % JT, k = r % JT;
+        const double v = fixed_decode(sJ[c * TL::JSC + a * TL::JSX + b * TL::JSY + k], quantum);
+        if (v == 0.) continue;
         const int gi = wrapi(cl.G0[0] - JO + a, g.N[0]);
         const int gj = wrapi(cl.G0[1] - JO + b, g.N[1]);
         const int gk = wrapi(cl.G0[2] - JO + k, g.N[2]);
@@ -531,8 +552,8 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 }
 
 // One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
-// Order 2: at most 128 VGPRs and 40,288 B of LDS, so that four wavefronts
-// per SIMD are resident (four 256-thread workgroups per CU).
+// Order 2: at most 128 VGPRs and 40 KB of LDS, so that four wavefronts per
+// SIMD are resident (four 256-thread workgroups per CU).
 template <int ORDER>
 __global__ void __launch_bounds__(512, ORDER == 2 ? 4 : 2)
 k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
