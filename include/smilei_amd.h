@@ -139,8 +139,10 @@ int smi_init_species_profile_mj(smi_context *ctx, int species, const double *den
                                 double theta, double drift_ux, uint64_t seed);
 int64_t smi_num_particles(const smi_context *ctx, int species);
 /* Particles in canonical order (patch-major, then cell key, then arrival
- * order); patch and key receive the owning patch id ((px*npy+py)*npz+pz)
- * and the in-patch cell key.  Any pointer may be NULL. */
+ * order; in the mixed form, whose storage is binned by the nearest grid
+ * node, the particles of a cell come by position, x then y then z); patch
+ * and key receive the owning patch id ((px*npy+py)*npz+pz) and the in-patch
+ * cell key of the cell the position lies in.  Any pointer may be NULL. */
 int smi_get_particles(smi_context *ctx, int species, double *x, double *y, double *z,
                       double *px, double *py, double *pz, double *w,
                       int32_t *patch, int32_t *key);

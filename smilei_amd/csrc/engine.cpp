@@ -876,11 +876,62 @@ int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, do
     std::vector<int32_t> perm(n);
     std::vector<uint32_t> keys(n);
     std::vector<double> tmp(n);
-    if (n) {
+    double *dst[7] = {x, y, z, px, py, pz, w};
+    if (c->mixed) {
+        // FP64 positions and the keys of the cells they lie in (the mixed
+        // form bins by node), then a counting sort by cell: cell-major order
+        DevBuf<double> t[7];
+        DevBuf<uint32_t> ck;
+        for (auto &q : t) q.alloc(n);
+        ck.alloc(n);
+        launch_from_mixed(c->g, S.mview(c->cur), S.keys[c->cur], n,
+                          PartPtrs{t[0].p, t[1].p, t[2].p, t[3].p, t[4].p, t[5].p, t[6].p}, ck.p, c->st);
+        DevBuf<int32_t> cnt, off, pm, bsum;
+        cnt.alloc(c->g.ncells);
+        off.alloc(c->g.ncells + 1);
+        pm.alloc(n);
+        SortBufs tb{};
+        tb.counts = cnt.p;
+        tb.cell_off = off.p;
+        tb.perm = pm.p;
+        tb.nbsum = sort_bsum_size(c->g.ncells);
+        bsum.alloc(tb.nbsum);
+        tb.bsum = bsum.p;
+        launch_fill_i32(cnt.p, c->g.ncells, 0, c->st);
+        launch_sort(c->g, ck.p, n, tb, c->st, false, true, false, nullptr);
+        SMI_HIP(hipStreamSynchronize(c->st));
+        if (n) {
+            SMI_HIP(hipMemcpy(perm.data(), pm.p, n * 4, hipMemcpyDeviceToHost));
+            SMI_HIP(hipMemcpy(keys.data(), ck.p, n * 4, hipMemcpyDeviceToHost));
+            // inside a cell, by position (x, then y, then z): an order that
+            // depends on the state only, not on the buffer history
+            std::vector<double> px(n), py(n), pz(n);
+            SMI_HIP(hipMemcpy(px.data(), t[0].p, n * sizeof(double), hipMemcpyDeviceToHost));
+            SMI_HIP(hipMemcpy(py.data(), t[1].p, n * sizeof(double), hipMemcpyDeviceToHost));
+            SMI_HIP(hipMemcpy(pz.data(), t[2].p, n * sizeof(double), hipMemcpyDeviceToHost));
+            const uint32_t m27 = (1u << 27) - 1;
+            for (int64_t a = 0; a < n;) {
+                int64_t b = a + 1;
+                while (b < n && (keys[perm[b]] & m27) == (keys[perm[a]] & m27)) ++b;
+                if (b - a > 1)
+                    std::sort(perm.begin() + a, perm.begin() + b, [&](int32_t u, int32_t v) {
+                        if (px[u] != px[v]) return px[u] < px[v];
+                        if (py[u] != py[v]) return py[u] < py[v];
+                        return pz[u] < pz[v];
+                    });
+                a = b;
+            }
+        }
+        for (int q = 0; q < 7; ++q) {
+            if (!dst[q] || !n) continue;
+            SMI_HIP(hipMemcpy(tmp.data(), t[q].p, n * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < n; ++j) dst[q][j] = tmp[perm[j]];
+        }
+    } else if (n) {
         SMI_HIP(hipMemcpy(perm.data(), S.sb.perm, n * 4, hipMemcpyDeviceToHost));
         SMI_HIP(hipMemcpy(keys.data(), S.keys[c->cur], n * 4, hipMemcpyDeviceToHost));
     }
-    if (S.lay && n) {
+    if (!c->mixed && S.lay && n) {
         // back to cell-major order: rank p of a cluster sits at slot cluster_slot(p)
         std::vector<int32_t> off(c->g.ncells + 1), lay(perm);
         SMI_HIP(hipMemcpy(off.data(), S.sb.cell_off, off.size() * 4, hipMemcpyDeviceToHost));
@@ -890,18 +941,7 @@ int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, do
             for (int p = 0; p < m; ++p) perm[a + p] = lay[a + cluster_slot(p, m)];
         }
     }
-    double *dst[7] = {x, y, z, px, py, pz, w};
-    if (c->mixed) {
-        // FP64 global positions in sorted order, built on the device
-        std::vector<double *> t(7, nullptr);
-        for (auto &q : t) q = dalloc<double>(n);
-        launch_from_mixed(c->g, S.mview(c->cur), S.keys[c->cur], S.sb.perm, S.sb.cell_off, S.lay, n,
-                          PartPtrs{t[0], t[1], t[2], t[3], t[4], t[5], t[6]}, c->st);
-        SMI_HIP(hipStreamSynchronize(c->st));
-        for (int q = 0; q < 7; ++q)
-            if (dst[q] && n) SMI_HIP(hipMemcpy(dst[q], t[q], n * sizeof(double), hipMemcpyDeviceToHost));
-        for (auto q : t) (void)hipFree(q);
-    } else {
+    if (!c->mixed) {
         for (int q = 0; q < 7; ++q) {
             if (!dst[q]) continue;
             if (q == 2 && c->g.nd < 3) {
@@ -929,6 +969,15 @@ int smi_patch_counts(smi_context *c, int s, int32_t *counts)
     if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
     SMI_HIP(hipSetDevice(c->p.device));
     SMI_HIP(hipStreamSynchronize(c->st));
+    if (c->mixed) {  // binned by node: count by the cells the positions lie in
+        DevBuf<int32_t> d;
+        d.alloc(c->g.npatches);
+        const SpeciesState &S = c->sp[s];
+        launch_patch_counts_mixed(c->g, S.mview(c->cur), S.keys[c->cur], S.n, d.p, c->st);
+        SMI_HIP(hipStreamSynchronize(c->st));
+        SMI_HIP(hipMemcpy(counts, d.p, c->g.npatches * sizeof(int32_t), hipMemcpyDeviceToHost));
+        return SMI_OK;
+    }
     std::vector<int32_t> off(c->g.ncells + 1);
     SMI_HIP(hipMemcpy(off.data(), c->sp[s].sb.cell_off, off.size() * 4, hipMemcpyDeviceToHost));
     for (int p = 0; p < c->g.npatches; ++p)

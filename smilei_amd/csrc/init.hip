@@ -8,10 +8,13 @@
 // (SplitMix64 finaliser of seed, species, particle and draw index), so the
 // result does not depend on the launch shape.
 #include "kernels.h"
+#include "pic_device.h"
 
 namespace smi {
 
 namespace {
+
+using dev::cluster_key3;
 
 __device__ inline uint64_t mix64(uint64_t z)
 {
@@ -76,20 +79,17 @@ __global__ void k_init(Geom g, PartPtrs A, MixPtrs M, uint32_t *__restrict__ key
     double u[3];
     draw_momentum(seed, i, sigma, uq, nq, drift, u);
     if (M.ox) {
-        // split-precision form: exact cell, FP32 offset of the same draw
-        int pc[3], l[3];
-        for (int d = 0; d < 3; ++d) {
-            pc[d] = cc[d] / g.n[d];
-            l[d] = cc[d] - pc[d] * g.n[d];
-        }
-        const int p = (pc[0] * g.npatch[1] + pc[1]) * g.npatch[2] + pc[2];
-        const int cl = ((l[0] >> 2) * g.ncl[1] + (l[1] >> 2)) * g.ncl[2] + (l[2] >> 2);
-        keys[i] = (13u << 27) | (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
+        // split-precision form: exact nearest node, FP32 offset of the same
+        // draw (f in the upper half of the cell belongs to the next node)
+        int nd[3];
         float fo[3];
         for (int d = 0; d < 3; ++d) {
-            fo[d] = (float)f[d];
-            if (fo[d] >= 1.f) fo[d] = 0.99999994f;
+            const int up = f[d] >= 0.5 ? 1 : 0;
+            nd[d] = wrapi(cc[d] + up, g.N[d]);
+            fo[d] = (float)(f[d] - (double)up);
+            if (fo[d] >= 0.5f) fo[d] = 0.49999997f;  // FP32 rounding of f just below 1/2
         }
+        keys[i] = (13u << 27) | cluster_key3(g, nd);
         M.ox[i] = fo[0];
         M.oy[i] = fo[1];
         M.oz[i] = fo[2];

@@ -143,8 +143,10 @@ void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &s
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
 
 // ---- mixed-precision form (particles_mixed.hip) -------------------------
-// Same contract as launch_particles (always the atomic tile deposit); the
-// cell of particle i is keysA[i], its new key goes to keysB[slot].
+// Same contract as launch_particles (always the atomic tile deposit), with
+// particles binned by their nearest grid node instead of their cell: the
+// node of particle i is keysA[i] (cell-key layout), its offsets from it are
+// in [-1/2, 1/2); its new key goes to keysB[slot].
 // One species of a merged particle launch of the mixed form (device array).
 struct MixSpecies {
     SpeciesKM k;
@@ -167,10 +169,13 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
 // Resident workgroups of the persistent mixed kernel on the current device.
 int mixed_persistent_blocks(const Geom &g, int threads);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
-// FP64 particles in cell-major sorted order (cell_off/interleaved: the
-// layout of perm, see launch_sort).
-void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,
-                       const int32_t *cell_off, bool interleaved, int64_t n, const PartPtrs &X, hipStream_t st);
+// FP64 global positions of the mixed particles in buffer order, and the
+// key (direction 13) of the cell each lies in.
+void launch_from_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, int64_t n, const PartPtrs &X,
+                       uint32_t *cell_keys, hipStream_t st);
+// Particles per patch (counts[npatches]) by the cell each position lies in.
+void launch_patch_counts_mixed(const Geom &g, const MixPtrs &M, const uint32_t *keys, int64_t n, int32_t *counts,
+                               hipStream_t st);
 void launch_ukin_mixed(const MixPtrs &M, const int32_t *perm, int64_t n, double *partials, int nblocks,
                        hipStream_t st);
 void launch_rho_mixed(const Geom &g, const SpeciesKM &sp, const MixPtrs &M, const uint32_t *keys, const int32_t *perm,

@@ -61,6 +61,37 @@ __device__ inline Cluster decode_cluster(const Geom &g, int b)
     return c;
 }
 
+// Global key (patch * cells-per-patch + cluster * 64 + 16 i + 4 j + k, the
+// layout of the cell keys) of global grid point c[3] (already wrapped).
+__device__ inline uint32_t cluster_key3(const Geom &g, const int *c)
+{
+    int pc[3], l[3];
+    for (int d = 0; d < 3; ++d) {
+        pc[d] = c[d] / g.n[d];
+        l[d] = c[d] - pc[d] * g.n[d];
+    }
+    const int p = (pc[0] * g.npatch[1] + pc[1]) * g.npatch[2] + pc[2];
+    const int cl = ((l[0] >> 2) * g.ncl[1] + (l[1] >> 2)) * g.ncl[2] + (l[2] >> 2);
+    return (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
+}
+
+// Mixed form: position xn = x/dx (grid units) -> nearest grid node, wrapped
+// into [0, N), and the FP32 offset from it in [-1/2, 1/2).
+__device__ inline void split_node(double xn, int N, int &node, float &off)
+{
+    const double fl = floor(xn + 0.5);
+    float f = (float)(xn - fl);
+    int c = (int)fl;
+    if (f >= 0.5f) {  // rounded up to 1/2 in FP32
+        f -= 1.f;
+        c += 1;
+    }
+    if (f < -0.5f) f = -0.5f;
+    c %= N;
+    node = c < 0 ? c + N : c;
+    off = f;
+}
+
 // In-patch cell key (Cluster3D::CellKey, nvidiaParticles.cu:639-660) of a
 // position inside patch `pc`; clamped against rounding at the upper face.
 template <int ND>
