@@ -48,7 +48,7 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
-TRAFFIC_PER_PUSH = {"beam": {"mixed": 189.0, "fp64": 392.2}}
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 127.9, "fp64": 179.4}}
 
 
 def cfl_dt(dx, factor=0.95):
@@ -216,9 +216,10 @@ def main():
                          energy_change=float(e1.sum() / e0.sum() - 1), energies=g["energies"])
     head = forms[0]
     h = res[head]
-    # HBM bytes per particle-kernel launch (a step launches it once per species)
+    # HBM bytes per particle-kernel launch (one launch per step runs every
+    # species of the workload)
     tpp = TRAFFIC_PER_PUSH.get(W["name"], {}).get(head)
-    traffic = tpp * h["npart"] / len(W["species"]) if tpp else None
+    traffic = tpp * h["npart"] if tpp else None
     out = {
         "metric": "particle-pushes/s (gather+Boris+Esirkepov) 1xMI355X, mixed vs FP64; %HBM peak",
         "value": h["value"],
