@@ -154,6 +154,14 @@ int smi_step(smi_context *ctx, int nsteps);
 int smi_synchronize(smi_context *ctx);
 /* out[0] = Uelm, out[1+s] = Ukin of species s (DiagnosticScalar formulas). */
 int smi_energies(smi_context *ctx, double *out);
+/* The per-patch sums of DiagnosticScalar::compute
+ * (src/Diagnostic/DiagnosticScalar.cpp:522-660) over the whole domain:
+ * out[0..5] = Uelm_Ex, Uelm_Ey, Uelm_Ez, Uelm_Bx_m, Uelm_By_m, Uelm_Bz_m
+ * (0.5 * V_cell * sum of squares, Field3D::norm2), then for each species s
+ * out[6+3s..] = Ntot (particles), Dens (sum of weights), Ukin (mass * sum
+ * w (gamma - 1)).  The energy balance (Utot, Uexp, Ubal) is formed by the
+ * caller from these, as SmileiMPI::computeGlobalDiags does. */
+int smi_scalars(smi_context *ctx, double *out);
 /* Charge density (all species) and div(E) - rho on primal nodes. */
 int smi_rho(smi_context *ctx, double *global);
 int smi_gauss(smi_context *ctx, double *global);

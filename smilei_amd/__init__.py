@@ -51,6 +51,7 @@ class Simulation:
                           interpolator=interpolator, pusher=pusher, maxwell_solver=maxwell_solver)
         self._species_args = []
         self.steps_done = 0
+        self.extra = {}
         nd = 2 if geometry == "2Dcartesian" else 3
         p = SmiParams()
         self._keep = [s.encode() for s in (geometry, interpolator, pusher, maxwell_solver)]
@@ -177,11 +178,12 @@ class Simulation:
     # ---- checkpoint / restart (the reference dumps HDF5, Checkpoint.cpp:212-587;
     # here a NumPy archive of the same state: every field array, every
     # species' particles in canonical order, the step count)
-    def checkpoint(self, path):
-        """Write the full simulation state to `path` (.npz)."""
+    def checkpoint(self, path, extra=None):
+        """Write the full simulation state to `path` (.npz); `extra`: a JSON-able
+        dict stored with it (returned as `.extra` by `restart`)."""
         import json
         data = {"meta": np.array(json.dumps(dict(args=self._args, species=self._species_args,
-                                                 steps_done=self.steps_done)))}
+                                                 steps_done=self.steps_done, extra=extra or {})))}
         for f in FIELDS:
             data["field_" + f] = self.get_field(f)
         for s in range(len(self.species)):
@@ -206,6 +208,7 @@ class Simulation:
             for f in FIELDS:
                 sim.set_field(f, z["field_" + f])
             sim.steps_done = int(meta["steps_done"])
+            sim.extra = meta.get("extra", {})
         return sim
 
     def synchronize(self):
@@ -221,6 +224,18 @@ class Simulation:
         out = np.empty(1 + len(self.species))
         check(self._L.smi_energies(self.h, _dp(out)))
         return out
+
+    def scalars(self):
+        """Per-field and per-species sums of DiagnosticScalar (see
+        smi_scalars): {"Uelm_Ex": .., ..., "Uelm_Bz_m": .., "Ntot": [..],
+        "Dens": [..], "Ukin": [..]} (lists over species)."""
+        ns = len(self.species)
+        out = np.empty(6 + 3 * ns)
+        check(self._L.smi_scalars(self.h, _dp(out)))
+        d = {"Uelm_" + f: float(out[k]) for k, f in enumerate(("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m"))}
+        for k, key in enumerate(("Ntot", "Dens", "Ukin")):
+            d[key] = [float(out[6 + 3 * s + k]) for s in range(ns)]
+        return d
 
     def rho(self):
         a = np.empty(self.shape)

@@ -61,6 +61,7 @@ SIGNATURES = {
     "smi_step": (ctypes.c_int, [_VP, ctypes.c_int]),
     "smi_synchronize": (ctypes.c_int, [_VP]),
     "smi_energies": (ctypes.c_int, [_VP, _D]),
+    "smi_scalars": (ctypes.c_int, [_VP, _D]),
     "smi_rho": (ctypes.c_int, [_VP, _D]),
     "smi_gauss": (ctypes.c_int, [_VP, _D]),
     "smi_profile_step": (ctypes.c_int, [_VP, _F]),

@@ -1184,6 +1184,49 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     SMI_CATCH
 }
 
+namespace {
+
+double reduce_partials(smi_context *c)
+{
+    std::vector<double> h(kReduceBlocks);
+    SMI_HIP(hipMemcpyAsync(h.data(), c->partials, kReduceBlocks * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    SMI_HIP(hipStreamSynchronize(c->st));
+    double s = 0;
+    for (double v : h) s += v;
+    return s;
+}
+
+}  // namespace
+
+int smi_scalars(smi_context *c, double *out)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (!out) throw Error(SMI_ERR_INVALID, "null output");
+    SMI_HIP(hipSetDevice(c->p.device));
+    double cell_volume = 1.0;
+    for (int d = 0; d < c->g.nd; ++d) cell_volume *= c->g.dx[d];
+    const int fid[6] = {SMI_EX, SMI_EY, SMI_EZ, SMI_BXM, SMI_BYM, SMI_BZM};
+    for (int f = 0; f < 6; ++f) {
+        const double *one[1] = {c->F[fid[f]]};
+        launch_sumsq(c->g, one, 1, c->partials, kReduceBlocks, c->st);
+        out[f] = 0.5 * cell_volume * reduce_partials(c);
+    }
+    for (size_t k = 0; k < c->sp.size(); ++k) {
+        SpeciesState &S = c->sp[k];
+        double *o = out + 6 + 3 * k;
+        o[0] = (double)S.n;
+        if (c->mixed) launch_wsum(nullptr, (const float *)S.buf[c->cur][6], S.n, c->partials, kReduceBlocks, c->st);
+        else launch_wsum((const double *)S.buf[c->cur][6], nullptr, S.n, c->partials, kReduceBlocks, c->st);
+        o[1] = reduce_partials(c);
+        if (c->mixed) launch_ukin_mixed(S.mview(c->cur), S.sb.perm, S.n, c->partials, kReduceBlocks, c->st);
+        else launch_ukin(S.view(c->cur), S.sb.perm, S.n, c->partials, kReduceBlocks, c->st);
+        o[2] = reduce_partials(c) * S.mass;
+    }
+    return SMI_OK;
+    SMI_CATCH
+}
+
 int smi_energies(smi_context *c, double *out)
 {
     SMI_TRY

@@ -160,6 +160,16 @@ __global__ void k_sumsq(Geom g, FieldList L, double *partials)
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// Sum of the weights w[0..n) (FP64 array w, or FP32 array wf).
+__global__ void k_wsum(const double *__restrict__ w, const float *__restrict__ wf, int64_t n, double *partials)
+{
+    double s = 0.;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        s += w ? w[j] : (double)wf[j];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 __global__ void k_ukin(PartPtrs A, const int32_t *__restrict__ perm, int64_t n, double *partials)
 {
     double s = 0.;
@@ -255,6 +265,12 @@ void launch_sumsq(const Geom &g, const double *const *fields, int nf, double *pa
     L.nf = nf;
     for (int c = 0; c < nf; ++c) L.f[c] = fields[c];
     hipLaunchKernelGGL(k_sumsq, dim3(nblocks), dim3(256), 0, st, g, L, partials);
+    SMI_LAUNCH_CHECK();
+}
+
+void launch_wsum(const double *w, const float *wf, int64_t n, double *partials, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wsum, dim3(nblocks), dim3(256), 0, st, w, wf, n, partials);
     SMI_LAUNCH_CHECK();
 }
 

@@ -207,6 +207,8 @@ void launch_sumsq(const Geom &g, const double *const *fields, int nf, double *pa
                   hipStream_t st);
 void launch_ukin(const PartPtrs &A, const int32_t *perm, int64_t n, double *partials, int nblocks,
                  hipStream_t st);
+// Sum of n weights (FP64 w or FP32 wf), per-block partials.
+void launch_wsum(const double *w, const float *wf, int64_t n, double *partials, int nblocks, hipStream_t st);
 void launch_rho(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm, int64_t n,
                 double *rho, hipStream_t st);
 void launch_gauss(const Geom &g, const FieldSet &f, double *rho_to_residual, hipStream_t st);

@@ -25,15 +25,25 @@ Supported (the periodic, Yee, Boris, momentum-conserving hot path):
   earlier species (copy of its positions), periodic boundary_conditions,
   pusher "boris";
 * ``ExternalField`` (initial fields, applied as applyExternalFields does);
-* ``DiagScalar`` (Uelm, Ukin and per-species Ukin every ``every`` steps).
+* ``DiagScalar``: ``scalars.txt`` in the reference's text layout (every,
+  vars, precision; the energy balance, per-species and per-field scalars;
+  no field min/max or Poynting columns);
+* ``Checkpoints``: dump_step, dump_minutes, exit_after_dump, keep_n_dumps,
+  restart_dir, and the signals SIGUSR1 (dump and stop) / SIGUSR2 (dump and
+  go on) of the reference (src/Checkpoint/Checkpoint.cpp:172-210); dumps
+  are NumPy archives (``checkpoints/dump-NNNNN.npz``), not HDF5.
 
-Blocks that change nothing on this path (load balancing, vectorisation,
-checkpoints and the other diagnostics) are accepted and recorded; blocks
+Blocks that change nothing on this path (load balancing, vectorisation and
+the other diagnostics) are accepted and recorded; blocks
 that would change the physics (lasers, moving window, collisions,
 injectors, filters, antennas, walls, radiation) raise ``NamelistError``
 rather than run a different simulation.
 """
+import glob
 import math
+import os
+import signal
+import time
 
 import numpy as np
 
@@ -81,7 +91,12 @@ class _DiagScalar(_Block):
     defaults = dict(every=1, vars=[], precision=10)
 
 
-_IGNORED = ("LoadBalancing", "Vectorization", "Checkpoints", "DiagFields", "DiagProbe", "DiagParticleBinning",
+class _Checkpoints(_Block):
+    defaults = dict(restart_dir=None, restart_number=None, dump_step=0, dump_minutes=0.0, keep_n_dumps=2,
+                    exit_after_dump=True, dump_deflate=0, file_grouping=None)
+
+
+_IGNORED = ("LoadBalancing", "Vectorization", "DiagFields", "DiagProbe", "DiagParticleBinning",
             "DiagScreen", "DiagTrackParticles", "DiagPerformances", "DiagRadiationSpectrum", "DiagNewParticles",
             "MultipleDecomposition")
 _REFUSED = ("Laser", "LaserEnvelope", "LaserGaussian2D", "LaserGaussian3D", "LaserPlanar1D", "LaserOffset",
@@ -98,6 +113,7 @@ class Namelist:
         self.species = []
         self.external_fields = []
         self.diag_scalar = None
+        self.checkpoints = None
         self.ignored = []
 
     # ---- derived parameters (after validation)
@@ -122,8 +138,11 @@ class Namelist:
         def DiagScalar(**kw):
             nl.diag_scalar = _DiagScalar(**kw)
 
+        def Checkpoints(**kw):
+            nl.checkpoints = _Checkpoints(**kw)
+
         ns = {"Main": Main, "Species": Species, "ExternalField": ExternalField, "DiagScalar": DiagScalar,
-              "math": math, "__name__": "__namelist__"}
+              "Checkpoints": Checkpoints, "math": math, "__name__": "__namelist__"}
         for name in _IGNORED:
             ns[name] = (lambda name: lambda *a, **kw: nl.ignored.append(name))(name)
         for name in _REFUSED:
@@ -202,7 +221,94 @@ class Namelist:
         for e in self.external_fields:
             if e.field not in P.DUAL or e.field.startswith("J"):
                 raise NamelistError("ExternalField: field %r is not an E, B or B_m component" % e.field)
+        if self.diag_scalar is not None:
+            d = self.diag_scalar
+            if int(d.every) < 1:
+                raise NamelistError("DiagScalar.every must be a positive number of steps")
+            known = set(scalar_names([s.name for s in self.species]))
+            bad = [v for v in d.vars if v not in known]
+            if bad:
+                raise NamelistError("DiagScalar.vars %s: not computed here (field min/max and Poynting "
+                                    "scalars are not part of this library)" % bad)
+        c = self.checkpoints
+        if c is not None:
+            if int(c.keep_n_dumps) < 1:
+                raise NamelistError("Checkpoints.keep_n_dumps must be >= 1")
+            if c.restart_number is not None:
+                raise NamelistError("Checkpoints.restart_number is not supported (the latest dump is used)")
         return self
+
+
+# DiagnosticScalar's scalars, in the order of DiagnosticScalar::init
+# (src/Diagnostic/DiagnosticScalar.cpp:249-286); the field min/max and
+# Poynting scalars that follow there are not computed here.
+_GENERAL = ["Ubal_norm", "Ubal", "Utot", "Uexp", "Ukin", "Urad", "UmBWpairs", "Uelm", "Ukin_bnd", "Ukin_out_mvw",
+            "Ukin_inj_mvw", "Ukin_new", "Uelm_bnd", "Uelm_out_mvw", "Uelm_inj_mvw"]
+_UELM = ["Uelm_Ex", "Uelm_Ey", "Uelm_Ez", "Uelm_Bx_m", "Uelm_By_m", "Uelm_Bz_m"]
+
+
+def scalar_names(species):
+    names = list(_GENERAL)
+    for s in species:
+        names += ["Dens_" + s, "Ntot_" + s, "Zavg_" + s, "Ukin_" + s, "Urad_" + s]
+    return names + _UELM
+
+
+def scalar_values(sim, charges, names, e_time_zero):
+    """The scalars of `names` for the current state, with the energy
+    balance of SmileiMPI::computeGlobalDiags (SmileiMPI.cpp:2366-2424):
+    Utot = Ukin + Uelm + Urad, Uexp = Utot at t = 0 plus the boundary and
+    injection terms (zero with periodic boundaries), Ubal = Utot - Uexp,
+    Ubal_norm = Ubal / Utot; Zavg = sum(w q) / sum(w)."""
+    d = sim.scalars()
+    v = dict.fromkeys(_GENERAL, 0.0)
+    for f in _UELM:
+        v[f] = d[f]
+    v["Uelm"] = sum(d[f] for f in _UELM)
+    v["Ukin"] = sum(d["Ukin"])
+    v["Utot"] = v["Ukin"] + v["Uelm"] + v["Urad"]
+    e0 = v["Utot"] if e_time_zero is None else e_time_zero
+    v["Uexp"] = e0
+    v["Ubal"] = v["Utot"] - v["Uexp"]
+    v["Ubal_norm"] = v["Ubal"] / v["Utot"] if v["Utot"] > 0 else 0.0
+    for k, s in enumerate(sim.species):
+        dens = d["Dens"][k]
+        v["Dens_" + s] = dens
+        v["Ntot_" + s] = d["Ntot"][k]
+        v["Zavg_" + s] = charges[k] * dens / dens if dens else float("nan")
+        v["Ukin_" + s] = d["Ukin"][k]
+        v["Urad_" + s] = 0.0
+    return [v[n] for n in names], e0
+
+
+class ScalarFile:
+    """``scalars.txt`` in the layout of DiagnosticScalar::write
+    (src/Diagnostic/DiagnosticScalar.cpp:350-397): a "# j name" line per
+    column, a "#" line, the names in one line, then one row per recorded
+    step, numbers in scientific notation with `precision` digits, each
+    column right-aligned in 2 + max(len(name), precision + 8) characters."""
+
+    def __init__(self, path, names, precision=10, append=False):
+        self.names, self.p = names, int(precision)
+        self.f = open(path, "a" if append else "w")
+        if self.f.tell() == 0:
+            self.f.write("# 1 time\n")
+            for j, n in enumerate(names):
+                self.f.write("# %d %s\n" % (j + 2, n))
+            self.f.write("#\n#" + "time".rjust(self.p + 9)
+                         + "".join(n.rjust(self.width(n)) for n in names) + "\n")
+
+    def width(self, name):
+        return 2 + max(len(name), self.p + 8)
+
+    def write(self, t, values):
+        fmt = "%%.%de" % self.p
+        self.f.write((fmt % t).rjust(self.p + 10)
+                     + "".join((fmt % v).rjust(self.width(n)) for n, v in zip(self.names, values)) + "\n")
+        self.f.flush()
+
+    def close(self):
+        self.f.close()
 
 
 def load(source):
@@ -263,31 +369,103 @@ def build(nl, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True):
     return sim
 
 
-def run(source, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, steps=None, scalars_path=None):
+def _latest_dump(restart_dir):
+    """The dump of restart_dir/checkpoints with the most steps done."""
+    import json
+    best, best_step = None, -1
+    for path in glob.glob(os.path.join(restart_dir, "checkpoints", "dump-*.npz")):
+        with np.load(path, allow_pickle=False) as z:
+            step = int(json.loads(str(z["meta"]))["steps_done"])
+        if step > best_step:
+            best, best_step = path, step
+    if best is None:
+        raise NamelistError("Checkpoints.restart_dir %r holds no dump" % restart_dir)
+    return best
+
+
+def run(source, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, steps=None, scalars_path=None, output_dir=".",
+        on_step=None):
     """Load, build and run a namelist for its number of time steps (or
-    `steps`); returns (simulation, scalars) where scalars is an array of rows
-    (time, Utot, Ukin, Uelm, Ukin_<species>...) every DiagScalar.every
-    steps (every step without a DiagScalar block), also written as text to
-    `scalars_path` when given."""
+    `steps`, counted from the start of this run).  Returns (simulation,
+    rows): rows = (time, Utot, Ukin, Uelm, Ukin_<species>...) at step 0 and
+    every DiagScalar.every steps (every step without a DiagScalar block).
+    With a DiagScalar block its scalars go to `scalars_path` (default
+    output_dir/scalars.txt) in the reference's layout; a Checkpoints block
+    dumps to output_dir/checkpoints and restarts from restart_dir.
+    SIGUSR1 during the run dumps and stops, SIGUSR2 dumps and goes on;
+    on_step(it), when given, is called after every step (progress, or
+    raising a signal at a chosen step)."""
     nl = load(source)
-    sim = build(nl, precision=precision, deposit=deposit)
-    n = nl.main.number_of_timesteps if steps is None else int(steps)
-    every = max(int(nl.diag_scalar.every), 1) if nl.diag_scalar is not None else 1
+    ck = nl.checkpoints
+    extra = {}
+    if ck is not None and ck.restart_dir:
+        sim = Simulation.restart(_latest_dump(ck.restart_dir), precision=precision, deposit=deposit)
+        extra = sim.extra
+    else:
+        sim = build(nl, precision=precision, deposit=deposit)
+    start = sim.steps_done
+    n_end = nl.main.number_of_timesteps if steps is None else start + int(steps)
+    dt = nl.main.timestep
+    diag = nl.diag_scalar
+    every = max(int(diag.every), 1) if diag is not None else 1
+    charges = [float(s.charge) for s in nl.species]
+    names = scalar_names(sim.species)
+    if diag is not None and diag.vars:
+        names = [n for n in names if n in diag.vars]
+    sfile = None
+    if diag is not None:
+        sfile = ScalarFile(scalars_path or os.path.join(output_dir, "scalars.txt"), names, diag.precision,
+                           append=start > 0)
+    e0 = extra.get("Energy_time_zero")
     rows = []
 
     def record(it):
+        nonlocal e0
         e = sim.energies()
-        rows.append([it * nl.main.timestep, float(e.sum()), float(e[1:].sum()), float(e[0])] + list(e[1:]))
+        rows.append([it * dt, float(e.sum()), float(e[1:].sum()), float(e[0])] + list(e[1:]))
+        if sfile is not None:
+            vals, e0 = scalar_values(sim, charges, names, e0)
+            sfile.write(it * dt, vals)
 
-    record(0)
-    done = 0
-    while done < n:
-        k = min(every, n - done)
-        sim.step(k)
-        done += k
-        record(done)
-    rows = np.array(rows)
-    if scalars_path:
-        head = "time Utot Ukin Uelm " + " ".join("Ukin_%s" % s.name for s in nl.species)
-        np.savetxt(scalars_path, rows, header=head)
-    return sim, rows
+    got = []
+    handlers = {}
+    for sig in (signal.SIGUSR1, signal.SIGUSR2):
+        try:
+            handlers[sig] = signal.signal(sig, lambda signum, frame: got.append(signum))
+        except ValueError:  # not the main thread: no signal triggers
+            pass
+    dump_number = int(extra.get("dump_number", 0))
+    t_ref = time.time()
+    try:
+        if start == 0:
+            record(0)  # a restarted run continues the file of the dumped one
+        it = start
+        while it < n_end:
+            sim.step(1)
+            it += 1
+            if on_step is not None:
+                on_step(it)
+            if it % every == 0:
+                record(it)
+            if ck is None and not got:
+                continue
+            due = got or (ck is not None and ((int(ck.dump_step) > 0 and (it - start) % int(ck.dump_step) == 0)
+                                              or (float(ck.dump_minutes) > 0
+                                                  and time.time() - t_ref > 60.0 * float(ck.dump_minutes))))
+            if due:
+                keep = int(ck.keep_n_dumps) if ck is not None else 2
+                os.makedirs(os.path.join(output_dir, "checkpoints"), exist_ok=True)
+                sim.checkpoint(os.path.join(output_dir, "checkpoints", "dump-%05d.npz" % (dump_number % keep)),
+                               extra={"Energy_time_zero": e0, "dump_number": dump_number + 1})
+                dump_number += 1
+                t_ref = time.time()
+                stop = (ck is None or bool(ck.exit_after_dump)) or (got and got[-1] != signal.SIGUSR2)
+                del got[:]
+                if stop:
+                    break
+    finally:
+        for sig, h in handlers.items():
+            signal.signal(sig, h)
+        if sfile is not None:
+            sfile.close()
+    return sim, np.array(rows)

@@ -8,6 +8,7 @@ from scipy.special import kve
 import smilei_amd as sa
 from smilei_amd import namelist as N
 from tests.plasma import cfl_dt
+from tests.test_namelist_cpu import happi_read
 
 pytestmark = pytest.mark.gpu
 
@@ -85,5 +86,74 @@ def test_namelist_thermal_plasma_runs_both_forms(tmp_path):
         sim.close()
     # same seeds, same initial state: mixed and FP64 agree
     np.testing.assert_allclose(out["mixed"][:, 1], out["fp64"][:, 1], rtol=1e-5)
-    text = (tmp_path / "scalars_fp64.txt").read_text().splitlines()[0]
-    assert text.split()[1:] == ["time", "Utot", "Ukin", "Uelm", "Ukin_proton", "Ukin_electron"]
+    # scalars.txt as happi reads it: same records, the reference's columns
+    nl = N.load(NAMELIST)
+    names, steps, v = happi_read(str(tmp_path / "scalars_fp64.txt"), nl.main.timestep)
+    assert steps == [0, 10, 20, 30, 40]
+    assert names == ["time"] + N.scalar_names(["proton", "electron"])
+    rows = out["fp64"]
+    np.testing.assert_allclose(v["Utot"], rows[:, 1], rtol=1e-10)  # 10 digits printed
+    np.testing.assert_allclose(v["Ukin_electron"], rows[:, 5], rtol=1e-10)
+    np.testing.assert_allclose(np.sum([v["Uelm_" + f] for f in ("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m")], 0),
+                               rows[:, 3], rtol=1e-10, atol=1e-300)
+    assert v["Uexp"] == [v["Utot"][0]] * 5
+    np.testing.assert_allclose(v["Ubal"], np.array(v["Utot"]) - v["Utot"][0], rtol=0, atol=1e-10 * v["Utot"][0])
+    assert v["Ntot_proton"] == [32 ** 3 * 8.0] * 5 and v["Zavg_electron"] == [-1.0] * 5
+    # sum of weights = density (1 / |charge|) x volume
+    vol = (32 * nl.main.cell_length[0]) ** 3
+    np.testing.assert_allclose(v["Dens_proton"], vol, rtol=1e-10)
+
+
+def test_checkpoint_dump_then_restart_continues_bit_for_bit(tmp_path):
+    """Checkpoints(dump_step, exit_after_dump): the run stops after its first
+    dump; a second run with restart_dir continues from it to the namelist's
+    end.  Deterministic FP64 deposit: the result equals an uninterrupted run
+    bit for bit, and scalars.txt holds each record once."""
+    base = NAMELIST.replace("DiagScalar(every = 10)", "DiagScalar(every = 5)")
+    ref_dir, a_dir, b_dir = tmp_path / "ref", tmp_path / "a", tmp_path / "b"
+    for d in (ref_dir, a_dir, b_dir):
+        d.mkdir()
+    ref, _ = N.run(base, deposit=sa.DEPOSIT_DETERMINISTIC, output_dir=str(ref_dir))
+    first, _ = N.run(base + "Checkpoints(dump_step = 15, exit_after_dump = True)\n",
+                     deposit=sa.DEPOSIT_DETERMINISTIC, output_dir=str(a_dir))
+    assert first.steps_done == 15 and (a_dir / "checkpoints" / "dump-00000.npz").exists()
+    first.close()
+    # the restarted run writes into the first run's directory (its scalars.txt continues)
+    second, _ = N.run(base + "Checkpoints(restart_dir = %r)\n" % str(a_dir),
+                      deposit=sa.DEPOSIT_DETERMINISTIC, output_dir=str(a_dir))
+    assert second.steps_done == ref.steps_done == 40
+    for s in range(2):
+        pa, pb = ref.get_particles(s), second.get_particles(s)
+        for k in ("x", "y", "z", "px", "py", "pz", "w"):
+            np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
+    for f in ("Ex", "By_m", "Jz"):
+        np.testing.assert_array_equal(ref.get_field(f), second.get_field(f))
+    nl = N.load(base)
+    names, steps, v = happi_read(str(a_dir / "scalars.txt"), nl.main.timestep)
+    _, steps_ref, v_ref = happi_read(str(ref_dir / "scalars.txt"), nl.main.timestep)
+    assert steps == steps_ref == list(range(0, 41, 5))
+    assert v == v_ref  # Uexp too: Energy_time_zero travels with the dump
+    ref.close()
+    second.close()
+
+
+def test_signals_dump_and_stop_or_go_on(tmp_path):
+    """SIGUSR2 dumps and goes on, SIGUSR1 dumps and stops (with
+    exit_after_dump = False), as Checkpoint::dump does."""
+    import signal
+
+    def raise_at(it):
+        if it == 5:
+            signal.raise_signal(signal.SIGUSR2)
+        if it == 9:
+            signal.raise_signal(signal.SIGUSR1)
+
+    text = NAMELIST + "Checkpoints(exit_after_dump = False, keep_n_dumps = 3)\n"
+    before = signal.getsignal(signal.SIGUSR1), signal.getsignal(signal.SIGUSR2)
+    sim, _ = N.run(text, output_dir=str(tmp_path), on_step=raise_at)
+    assert sim.steps_done == 9
+    dumps = sorted((tmp_path / "checkpoints").glob("dump-*.npz"))
+    assert [d.name for d in dumps] == ["dump-00000.npz", "dump-00001.npz"]
+    assert [sa.Simulation.restart(str(d)).steps_done for d in dumps] == [5, 9]
+    # the handlers in place before the run are back
+    assert (signal.getsignal(signal.SIGUSR1), signal.getsignal(signal.SIGUSR2)) == before

@@ -93,3 +93,93 @@ def test_species_validation():
     with pytest.raises(N.NamelistError, match="mean_velocity"):
         N.load(base + "Species(name='e', mass=1., charge=-1., particles_per_cell=4, number_density=1., "
                       "mean_velocity=[0., 0.1, 0.])\n")
+
+
+def happi_read(path, timestep):
+    """Read scalars.txt the way happi does (happi/_Diagnostics/Scalar.py:
+    64-85): the names are the last comment line, each row is split on
+    whitespace, the time column becomes a step number."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+    while lines[k].startswith("#"):
+        k += 1
+    names = lines[k - 1][1:].split()
+    rows = [[float(v) for v in L.split()] for L in lines[k:]]
+    steps = [int(round(r[0] / timestep)) for r in rows]
+    return names, steps, {n: [r[i] for r in rows] for i, n in enumerate(names)}
+
+
+def test_scalar_file_layout(tmp_path):
+    """The text layout of DiagnosticScalar::write: "# j name" lines, a "#"
+    line, the names right-aligned in 2 + max(len, precision + 8) columns
+    after a 1 + precision + 9 wide time column, then %.{precision}e values."""
+    path = tmp_path / "scalars.txt"
+    f = N.ScalarFile(str(path), ["Utot", "Ukin_electron"], precision=10)
+    f.write(0.5, [1.25, -3e-5])
+    f.write(1.0, [2.0, float("nan")])
+    f.close()
+    assert path.read_text().splitlines() == [
+        "# 1 time",
+        "# 2 Utot",
+        "# 3 Ukin_electron",
+        "#",
+        "#               time                Utot       Ukin_electron",
+        "    5.0000000000e-01    1.2500000000e+00   -3.0000000000e-05",
+        "    1.0000000000e+00    2.0000000000e+00                 nan",
+    ]
+    names, steps, v = happi_read(str(path), 0.25)
+    assert names == ["time", "Utot", "Ukin_electron"] and steps == [2, 4]
+    assert v["Utot"] == [1.25, 2.0]
+    # a long name widens its column; precision 3 narrows the others
+    f = N.ScalarFile(str(tmp_path / "s3.txt"), ["Uelm_Bx_m", "Ubal_norm"], precision=3)
+    f.write(0.0, [1.0, 0.0])
+    f.close()
+    lines = (tmp_path / "s3.txt").read_text().splitlines()
+    assert lines[-2] == "#        time    Uelm_Bx_m    Ubal_norm"
+    assert lines[-1] == "    0.000e+00    1.000e+00    0.000e+00"
+
+
+class _FakeSim:
+    species = ["ion", "electron"]
+
+    def __init__(self, uelm, ntot, dens, ukin):
+        self.v = dict(zip(N._UELM, uelm))
+        self.v.update(Ntot=ntot, Dens=dens, Ukin=ukin)
+
+    def scalars(self):
+        return self.v
+
+
+def test_scalar_energy_balance():
+    """Utot = Ukin + Uelm, Uexp = Utot at t = 0, Ubal = Utot - Uexp,
+    Ubal_norm = Ubal / Utot, Zavg = charge (SmileiMPI.cpp:2366-2424)."""
+    names = N.scalar_names(_FakeSim.species)
+    assert names[:3] == ["Ubal_norm", "Ubal", "Utot"] and names[15:20] == [
+        "Dens_ion", "Ntot_ion", "Zavg_ion", "Ukin_ion", "Urad_ion"]
+    assert names[-6:] == ["Uelm_Ex", "Uelm_Ey", "Uelm_Ez", "Uelm_Bx_m", "Uelm_By_m", "Uelm_Bz_m"]
+    s0 = _FakeSim([1., 2., 0., 0., 0., 1.], [10, 20], [5., 4.], [30., 6.])
+    v, e0 = N.scalar_values(s0, [1.0, -1.0], names, None)
+    d = dict(zip(names, v))
+    assert d["Uelm"] == 4.0 and d["Ukin"] == 36.0 and d["Utot"] == 40.0 and e0 == 40.0
+    assert d["Uexp"] == 40.0 and d["Ubal"] == 0.0 and d["Ubal_norm"] == 0.0
+    assert d["Zavg_ion"] == 1.0 and d["Zavg_electron"] == -1.0 and d["Ntot_electron"] == 20
+    s1 = _FakeSim([1., 2., 0., 0., 0., 3.], [10, 20], [5., 0.], [30., 4.])
+    v, _ = N.scalar_values(s1, [1.0, -1.0], names, e0)
+    d = dict(zip(names, v))
+    assert d["Utot"] == 40.0 and d["Ubal"] == 0.0
+    s2 = _FakeSim([0.] * 6, [10, 0], [5., 0.], [44., 0.])
+    v, _ = N.scalar_values(s2, [1.0, -1.0], names, e0)
+    d = dict(zip(names, v))
+    assert d["Ubal"] == 4.0 and d["Ubal_norm"] == 4.0 / 44.0
+    assert math.isnan(d["Zavg_electron"])  # no weight: 0/0, as the reference divides
+
+
+def test_diag_and_checkpoint_blocks_are_checked():
+    with pytest.raises(N.NamelistError, match="Poynting"):
+        N.load(THERMAL.replace("DiagScalar(every = 2)", "DiagScalar(every = 2, vars = ['Utot', 'PoyXmin'])"))
+    with pytest.raises(N.NamelistError, match="keep_n_dumps"):
+        N.load(THERMAL + "Checkpoints(dump_step = 5, keep_n_dumps = 0)\n")
+    nl = N.load(THERMAL + "Checkpoints(dump_step = 5, exit_after_dump = False)\n")
+    assert nl.checkpoints.dump_step == 5 and nl.checkpoints.exit_after_dump is False
+    assert nl.diag_scalar.every == 2
