@@ -1,0 +1,144 @@
+"""Edge cases of the engine: empty species, the capacity limit, very dense
+cells (the in-cell fix-up's insertion sort, the mixed kernel's node-changer
+queue overflowing into the inline deposit), particles exactly on cell faces
+and at the end of the box."""
+import numpy as np
+import pytest
+
+import smilei_amd as sa
+from oracle import pyoracle as O
+from tests.plasma import cfl_dt, load, thermal
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"]
+SHAPE, PATCH, DX = (16, 16, 16), (8, 8, 8), (0.2, 0.2, 0.2)
+
+
+def new(precision=sa.PLAIN_FP64, deposit=sa.DEPOSIT_ATOMIC, use_graph=True):
+    return sa.Simulation("3Dcartesian", 2, SHAPE, PATCH, DX, cfl_dt(DX), precision=precision, deposit=deposit,
+                         use_graph=use_graph)
+
+
+@pytest.mark.parametrize("precision", [sa.PLAIN_FP64, sa.MIXED])
+def test_empty_species_step_and_report(precision):
+    """A species without particles (capacity 0, and one emptied by loading
+    zero particles) steps with the others, graph-captured, and reports
+    nothing."""
+    sim = new(precision)
+    a = sim.add_species("electron", 1.0, -1.0, 16 ** 3 * 2)
+    load(sim, a, thermal(SHAPE, DX, 2, 0.1, 1))
+    b = sim.add_species("none", 1.0, -1.0, 0)
+    c = sim.add_species("emptied", 1836.0, 1.0, 100)
+    z = np.zeros(0)
+    sim.set_particles(c, z, z, z, z, z, z, z)
+    sim.step(4)
+    sim.synchronize()
+    for s in (b, c):
+        assert sim.num_particles(s) == 0
+        assert len(sim.get_particles(s)["x"]) == 0
+        assert sim.patch_counts(s).sum() == 0
+    e = sim.energies()
+    assert np.all(np.isfinite(e)) and e[2] == 0.0 and e[3] == 0.0
+    assert sim.num_particles(a) == 16 ** 3 * 2
+
+
+def test_capacity_is_enforced():
+    sim = new()
+    s = sim.add_species("e", 1.0, -1.0, 10)
+    p = thermal(SHAPE, DX, 1, 0.1, 2)
+    with pytest.raises(RuntimeError, match="capacity"):
+        load(sim, s, p)
+    with pytest.raises(RuntimeError):
+        sim.add_species("huge", 1.0, -1.0, 1 << 31)
+
+
+def dense_plasma(seed=5, n_dense=3000, sigma=0.3):
+    """A thin background plus n_dense particles in one cell (8, 8, 8), fast
+    enough that most of them change node every step."""
+    rng = np.random.default_rng(seed)
+    p = thermal(SHAPE, DX, 1, 0.05, seed)
+    c = np.array([8, 8, 8])
+    pos = (c + rng.random((n_dense, 3))) * np.array(DX)
+    u = rng.normal(0.0, sigma, (n_dense, 3))
+    w = np.full(n_dense, p["w"][0])
+    for k, key in enumerate(("x", "y", "z")):
+        p[key] = np.concatenate([p[key], pos[:, k]])
+    for k, key in enumerate(("px", "py", "pz")):
+        p[key] = np.concatenate([p[key], u[:, k]])
+    p["w"] = np.concatenate([p["w"], w])
+    return p
+
+
+def test_dense_cell_deterministic_bit_exact():
+    """3000 particles in one cell: buckets far beyond the register ranking
+    (insertion-sort fix-up) and a long per-patch deposit list; the
+    deterministic engine stays bit-identical to the oracle."""
+    p = dense_plasma()
+    g = new(deposit=sa.DEPOSIT_DETERMINISTIC)
+    o = O.OracleSim(3, 2, SHAPE, PATCH, DX, cfl_dt(DX))
+    load(g, g.add_species("e", 1.0, -1.0, len(p["x"])), p)
+    o.set_particles(o.add_species(1.0, -1.0), p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    g.step(3)
+    o.step(3)
+    a, b = g.get_particles(0), o.get_particles(0)
+    for k in ("x", "y", "z", "px", "py", "pz", "w", "key", "patch"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    for f in FIELDS:
+        np.testing.assert_array_equal(g.get_field(f), o.get_field(f), err_msg=f)
+
+
+def test_dense_cell_mixed_queue_overflow_conserves_charge():
+    """The mixed kernel defers node-changing particles to an LDS queue of a
+    few hundred entries; 3000 fast particles in one cluster overflow it and
+    the rest deposit inline.  Charge conservation (Gauss law) and the
+    energies against the FP64 form hold all the same."""
+    p = dense_plasma(sigma=0.5)
+    m, f = new(sa.MIXED), new()
+    for sim in (m, f):
+        load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
+    g0 = m.gauss()
+    rho0 = np.abs(m.rho()).max()
+    m.step(10)
+    f.step(10)
+    drift = np.abs(m.gauss() - g0).max() / rho0
+    assert drift < 1e-6, drift
+    em, ef = m.energies(), f.energies()
+    np.testing.assert_allclose(em.sum(), ef.sum(), rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", [sa.PLAIN_FP64, sa.MIXED])
+def test_positions_on_cell_faces_and_box_end(precision):
+    """Particles exactly on cell faces, on patch faces, at 0 and just below
+    the box length: binned in the cell they lie in, as the reference's
+    floor(x / dx) (FP64: keys equal the oracle's)."""
+    L = np.array(SHAPE) * np.array(DX)
+    vals = np.array([0.0, 1 * DX[0], 7 * DX[0], 8 * DX[0], 15 * DX[0], np.nextafter(L[0], 0.0), 0.5 * DX[0]])
+    g = np.stack(np.meshgrid(vals, vals, vals, indexing="ij"), -1).reshape(-1, 3)
+    n = len(g)
+    z = np.zeros(n)
+    w = np.full(n, 0.001)
+    sim = new(precision)
+    s = sim.add_species("e", 1.0, -1.0, n)
+    sim.set_particles(s, g[:, 0], g[:, 1], g[:, 2], z, z, z, w)
+    got = sim.get_particles(s)
+    pos = np.stack([got["x"], got["y"], got["z"]], 1)
+    # the reference's rule: floor(x * (1 / dx)) (nvidiaParticles.cu:639-660), clamped
+    cell = np.minimum(np.floor(pos * (1.0 / np.array(DX))).astype(int), np.array(SHAPE) - 1)
+    pc = cell // np.array(PATCH)
+    npc = np.array(SHAPE) // np.array(PATCH)
+    np.testing.assert_array_equal(got["patch"], (pc[:, 0] * npc[1] + pc[:, 1]) * npc[2] + pc[:, 2])
+    np.testing.assert_array_equal(sim.patch_counts(s), np.bincount(got["patch"], minlength=sim.npatch))
+    if precision == sa.PLAIN_FP64:
+        o = O.OracleSim(3, 2, SHAPE, PATCH, DX, cfl_dt(DX))
+        o.set_particles(o.add_species(1.0, -1.0), g[:, 0], g[:, 1], g[:, 2], z, z, z, w)
+        b = o.get_particles(0)
+        for k in ("x", "y", "z", "key", "patch"):
+            np.testing.assert_array_equal(got[k], b[k], err_msg=k)
+    else:
+        # the split representation keeps the positions to FP32 offsets
+        np.testing.assert_allclose(np.sort(pos.sum(1)), np.sort(g.sum(1)), rtol=0, atol=1e-6)
+    # at rest, nothing moves; one step later the bookkeeping is unchanged
+    sim.step(1)
+    after = sim.get_particles(s)
+    np.testing.assert_array_equal(after["patch"], got["patch"])
