@@ -451,6 +451,8 @@ int smi_destroy(smi_context *c)
         (void)hipFree(s.sb.perm);
         (void)hipFree(s.sb.tmp);
         (void)hipFree(s.sb.rank);
+        (void)hipFree(s.sb.big);
+        (void)hipFree(s.sb.nbig);
         (void)hipFree(s.sb.bsum);
         (void)hipFree(s.scratch);
     }
@@ -488,10 +490,10 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.sb.counts = dalloc<int32_t>(c->g.ncells);
     s.sb.cell_off = dalloc<int32_t>(c->g.ncells + 1);
     s.sb.perm = dalloc<int32_t>(capacity);
-    if (c->p.deposit != SMI_DEPOSIT_DETERMINISTIC) {
-        s.sb.tmp = dalloc<int32_t>(capacity);
-        s.sb.rank = dalloc<int32_t>(capacity);
-    }
+    s.sb.tmp = dalloc<int32_t>(capacity);
+    s.sb.big = dalloc<int32_t>(capacity / 17 + 1);
+    s.sb.nbig = dalloc<int32_t>(2);
+    if (c->p.deposit != SMI_DEPOSIT_DETERMINISTIC) s.sb.rank = dalloc<int32_t>(capacity);
     s.sb.nbsum = sort_bsum_size(c->g.ncells);
     s.sb.bsum = dalloc<int32_t>(s.sb.nbsum);
     if (c->p.deposit == SMI_DEPOSIT_DETERMINISTIC) s.scratch = dalloc<double>(7 * capacity);
@@ -506,7 +508,9 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.k.qmh = charge * one_over_mass * dts2;
     s.k.cw_scale = (1. / cell_volume) * charge;
     s.km.qmh = (float)s.k.qmh;
+    s.km.qmh_lo = (float)(s.k.qmh - (double)s.km.qmh);
     s.km.cw_scale = (float)s.k.cw_scale;
+    s.km.cwd = s.k.cw_scale;
     for (int d = 0; d < 3; ++d) s.km.dxdt[d] = (float)(c->g.dx[d] / c->g.dt);
     set_quantum(c->g, s, 1.0);
     c->sp.push_back(s);
@@ -898,7 +902,7 @@ int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, do
         bsum.alloc(tb.nbsum);
         tb.bsum = bsum.p;
         launch_fill_i32(cnt.p, c->g.ncells, 0, c->st);
-        launch_sort(c->g, ck.p, n, tb, c->st, false, true, false, nullptr);
+        launch_sort(c->g, ck.p, n, tb, c->st, false, false, false, nullptr);  // order inside cells: below
         SMI_HIP(hipStreamSynchronize(c->st));
         if (n) {
             SMI_HIP(hipMemcpy(perm.data(), pm.p, n * 4, hipMemcpyDeviceToHost));

@@ -20,9 +20,11 @@ struct MixPtrs {
 
 // Species constants of the mixed kernel (FP32).
 struct SpeciesKM {
-    float qmh;          // charge/mass * dt/2
-    float cw_scale;     // charge / V_cell
+    float qmh;          // charge/mass * dt/2, rounded to FP32
+    float qmh_lo;       // qmh_fp64 - qmh (the kick applies both)
+    float cw_scale;     // charge / V_cell, rounded to FP32
     float dxdt[3];      // dx/dt, dy/dt, dz/dt
+    double cwd;         // charge / V_cell in FP64 (current and density weights)
     double cr_max;      // largest current contribution |q| w_max / V * max(dx/dt)
 };
 
@@ -117,6 +119,8 @@ struct SortBufs {
     int32_t *perm;     // cap
     int32_t *tmp;      // cap (cell-major order before the interleaved layout)
     int32_t *rank;     // cap (tie-break of the canonical order: input position, cell-major)
+    int32_t *big;      // cap / (kFixRegs + 1) + 1, or null: cells too full for the register ranking
+    int32_t *nbig;     // 2 (count, queue head of `big`)
     int32_t *bsum;     // scan scratch
     int64_t nbsum;
 };

@@ -318,8 +318,8 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     // particles whose node changes on some axis: their deposit (the wider
     // stencil) is deferred to a second pass over this queue, so that the
     // wavefronts of the main loop run the narrow stencil with all lanes;
-    // entry = old and new deltas, charge weight in quanta, packed stencil
-    // origin and shifts
+    // entry = old and new deltas, weight, packed stencil origin, shifts and
+    // species
     constexpr int QCAP = TL::QCAP;
     __shared__ float sQ[QCAP][8];
     __shared__ int sQn;
@@ -343,7 +343,15 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     __syncthreads();
 
     const double quantum = kUnit * (double)total * cr_max, inv_quantum = 1.0 / quantum;
-    const float dtdx[3] = {(float)(g.dt * g.dinv[0]), (float)(g.dt * g.dinv[1]), (float)(g.dt * g.dinv[2])};
+    // dt/dx as an FP32 pair (hi + lo): the move applies both, so that the
+    // rounding of the constant does not drift every particle the same way
+    float dtdx[3], dtdx_lo[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double v = g.dt * g.dinv[d];
+        dtdx[d] = (float)v;
+        dtdx_lo[d] = (float)(v - (double)dtdx[d]);
+    }
     // cr_c = q w / V * dx_c / dt with one common factor: the ratio between
     // components must be exact for the discrete continuity equation
     const double dxdt[3] = {g.dx[0] / g.dt, g.dx[1] / g.dt, g.dx[2] / g.dt};
@@ -404,11 +412,14 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         Bf[2] = gatherf<H, GSX, GSY>(cd[0], cd[1], cp[2], sF + 5 * GF, d0, d1, p2);
 
         // ---- Boris push, FP32.  gamma from 1 + |u|^2 with the squares summed
-        // before adding 1 and one Newton step on rsqrt.
-        const float qmh = sp.qmh;
+        // before adding 1 and one Newton step on rsqrt.  The electric kicks
+        // apply q dt / 2m as an FP32 pair (hi + lo): the rounding of the
+        // constant would otherwise bias every kick of the species the same
+        // way (a systematic error, where the arithmetic's rounding is random).
+        const float qmh = sp.qmh, qmh_lo = sp.qmh_lo;
         float um[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) um[d] = fmaf(qmh, E[d], u[d]);
+        for (int d = 0; d < 3; ++d) um[d] = fmaf(qmh, E[d], fmaf(qmh_lo, E[d], u[d]));
         float g2 = 1.f + fmaf(um[0], um[0], fmaf(um[1], um[1], um[2] * um[2]));
         float ig = rsqrtf(g2);
         ig = ig * fmaf(-0.5f * g2 * ig, ig, 1.5f);
@@ -423,7 +434,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         float un[3] = {fmaf(up[1], sv[2], fmaf(-up[2], sv[1], um[0])), fmaf(up[2], sv[0], fmaf(-up[0], sv[2], um[1])),
                        fmaf(up[0], sv[1], fmaf(-up[1], sv[0], um[2]))};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) un[d] = fmaf(qmh, E[d], un[d]);
+        for (int d = 0; d < 3; ++d) un[d] = fmaf(qmh, E[d], fmaf(qmh_lo, E[d], un[d]));
         g2 = 1.f + fmaf(un[0], un[0], fmaf(un[1], un[1], un[2] * un[2]));
         ig = rsqrtf(g2);
         ig = ig * fmaf(-0.5f * g2 * ig, ig, 1.5f);
@@ -435,7 +446,8 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         int step[3], mdir[3], dpc[3], lp[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            float v = fmaf(un[d] * ig, dtdx[d], o[d]);
+            const float vel = un[d] * ig;
+            float v = fmaf(vel, dtdx[d], fmaf(vel, dtdx_lo[d], o[d]));
             int st = 0;
             if (v < -0.5f) {
                 v += 1.f;
@@ -466,7 +478,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
                 o3[d] = l[d];  // old node relative to G0 == tile coordinate of stencil index 0 (JO = H+1)
             }
             // contributions in units of the cluster's fixed-point quantum
-            const double cwq = (double)(sp.cw_scale * w) * inv_quantum;
+            const double cwq = sp.cwd * (double)w * inv_quantum;
             const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
             if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER>(sJ, dold, dnew, sh, o3, crq);
             // node changers: one queue slot range per wavefront
@@ -481,14 +493,14 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
                     const int pos = base + __popcll(gm & ((1ull << lane) - 1));
                     if (pos < QCAP) {
                         const int packed = o3[0] | (o3[1] << 3) | (o3[2] << 6) | ((sh[0] + 1) << 9) |
-                                           ((sh[1] + 1) << 11) | ((sh[2] + 1) << 13);
+                                           ((sh[1] + 1) << 11) | ((sh[2] + 1) << 13) | (sid << 15);
                         sQ[pos][0] = dold[0];
                         sQ[pos][1] = dold[1];
                         sQ[pos][2] = dold[2];
                         sQ[pos][3] = dnew[0];
                         sQ[pos][4] = dnew[1];
                         sQ[pos][5] = dnew[2];
-                        sQ[pos][6] = (float)(sp.cw_scale * w);
+                        sQ[pos][6] = w;
                         sQ[pos][7] = __int_as_float(packed);
                     } else {
                         deposit_fixed<ORDER, ORDER + 2, ORDER + 1>(sJ, dold, dnew, sh, o3, crq);
@@ -527,7 +539,8 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     for (int t = threadIdx.x; t < nq; t += blockDim.x) {
         const float dold[3] = {sQ[t][0], sQ[t][1], sQ[t][2]};
         const float dnew[3] = {sQ[t][3], sQ[t][4], sQ[t][5]};
-        const double cwq = (double)sQ[t][6] * inv_quantum;
+        const int packed_ = __float_as_int(sQ[t][7]);
+        const double cwq = specs[packed_ >> 15].k.cwd * (double)sQ[t][6] * inv_quantum;
         const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
         const int packed = __float_as_int(sQ[t][7]);
         const int o3[3] = {packed & 7, (packed >> 3) & 7, (packed >> 6) & 7};
@@ -719,7 +732,7 @@ __global__ void k_rho_mixed(Geom g, SpeciesKM sp, MixPtrs M, const uint32_t *__r
         for (int a = 0; a < W; ++a) S[d][a] = w[a];
         base[d] = c[d] - H;
     }
-    const double cwv = (double)sp.cw_scale * (double)M.w[i];
+    const double cwv = sp.cwd * (double)M.w[i];
     for (int a = 0; a < W; ++a)
         for (int b = 0; b < W; ++b)
             for (int k = 0; k < W; ++k) {

@@ -187,14 +187,22 @@ __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const in
 // One thread per cell.  Buckets of up to kFixRegs particles are ranked in
 // registers (all loads issued together, rank = number of smaller
 // (direction, slot) pairs; the pairs are distinct because slots are);
-// larger buckets fall back to an insertion sort in memory.  The cell-major
-// input is src; cpc == 0: sorted in place (dst == src); cpc > 0: every
-// particle is written to its lane-interleaved slot of the cluster in dst.
-// The tie-break value of source slot v is tie[v] when tie is given, else v.
+// larger buckets go to the work list `big` (count in nbig[0]) for
+// k_bucket_fix_big, or, without a list, to an insertion sort in memory.
+// The cell-major input is src; cpc == 0: sorted in place (dst == src);
+// cpc > 0: every particle is written to its lane-interleaved slot of the
+// cluster in dst.  The tie-break value of source slot v is tie[v] when tie
+// is given, else v.
 constexpr int kFixRegs = 16;
 
+__device__ inline uint64_t fix_key(const uint32_t *keys, const int32_t *tie, int32_t v)
+{
+    return ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)(tie ? tie[v] : v);
+}
+
 __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
-                             int32_t *src, int32_t *dst, int cpc, const int32_t *__restrict__ tie)
+                             int32_t *src, int32_t *dst, int cpc, const int32_t *__restrict__ tie,
+                             int32_t *__restrict__ big, int32_t *__restrict__ nbig)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncells) return;
@@ -239,6 +247,10 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
         }
         return;
     }
+    if (big) {
+        big[atomicAdd(&nbig[0], 1)] = (int32_t)g;
+        return;
+    }
     int32_t *perm = src;
     for (int i = a + 1; i < b; ++i) {
         const int32_t v = perm[i];
@@ -255,6 +267,107 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
     }
     if (cpc)
         for (int i = a; i < b; ++i) put(i, perm[i]);
+}
+
+// The cells of the work list (more than kFixRegs particles; their number in
+// nbig[0], nbig[1] = 0 on entry), one workgroup per cell pulled from the
+// list until it is exhausted.  Up to kBigLds particles: bitonic sort of the
+// (key, slot) pairs in LDS.  More: each particle's rank = the number of
+// smaller keys, counted over the cell in LDS-staged chunks, kBigLds target
+// particles at a time (in place, cpc == 0, the cell is first copied to scr
+// so that no write can overtake a read).  Every workgroup exits once the
+// list is exhausted.
+constexpr int kBigThreads = 256;
+constexpr int kBigLds = 4096;
+constexpr int kBigPer = kBigLds / kBigThreads;
+constexpr int kBigBlocks = 1024;
+
+__global__ void __launch_bounds__(kBigThreads)
+k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ cell_off, int32_t *src, int32_t *dst,
+                 int32_t *scr, int cpc, const int32_t *__restrict__ tie, const int32_t *__restrict__ big,
+                 int32_t *__restrict__ nbig)
+{
+    __shared__ uint64_t sK[kBigLds];
+    __shared__ int32_t sV[kBigLds];
+    __shared__ int sItem;
+    const int nb = nbig[0];
+    const int tid = threadIdx.x;
+    for (;;) {
+        __syncthreads();  // the previous cell is done with the LDS arrays
+        if (tid == 0) sItem = atomicAdd(&nbig[1], 1);
+        __syncthreads();
+        const int item = sItem;
+        if (item >= nb) break;
+        const int64_t g = big[item];
+        const int a = cell_off[g], n = cell_off[g + 1] - a;
+        int cs = 0, cn = 0;
+        if (cpc) {
+            const int64_t cb = g & ~(int64_t)(cpc - 1);
+            cs = cell_off[cb];
+            cn = cell_off[cb + cpc] - cs;
+        }
+        auto put = [&](int pos, int32_t v) { dst[cpc ? cs + cluster_slot(pos - cs, cn) : pos] = v; };
+        if (n <= kBigLds) {
+            int P = 1;
+            while (P < n) P <<= 1;
+            for (int i = tid; i < P; i += kBigThreads) {
+                const int32_t v = i < n ? src[a + i] : 0;
+                sK[i] = i < n ? fix_key(keys, tie, v) : ~0ull;
+                sV[i] = v;
+            }
+            __syncthreads();
+            for (int k = 2; k <= P; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = tid; i < P; i += kBigThreads) {
+                        const int l = i ^ j;
+                        if (l > i && ((sK[i] > sK[l]) == ((i & k) == 0))) {
+                            const uint64_t tk = sK[i];
+                            sK[i] = sK[l];
+                            sK[l] = tk;
+                            const int32_t tv = sV[i];
+                            sV[i] = sV[l];
+                            sV[l] = tv;
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (int i = tid; i < n; i += kBigThreads) put(a + i, sV[i]);
+            continue;
+        }
+        const int32_t *rd = src;
+        if (!cpc) {
+            for (int i = tid; i < n; i += kBigThreads) scr[a + i] = src[a + i];
+            __threadfence_block();
+            __syncthreads();
+            rd = scr;
+        }
+        for (int t0 = 0; t0 < n; t0 += kBigLds) {
+            uint64_t mk[kBigPer];
+            int32_t mv[kBigPer];
+            int rk[kBigPer];
+#pragma unroll
+            for (int m = 0; m < kBigPer; ++m) {
+                const int i = t0 + tid + m * kBigThreads;
+                mv[m] = i < n ? rd[a + i] : 0;
+                mk[m] = i < n ? fix_key(keys, tie, mv[m]) : ~0ull;
+                rk[m] = 0;
+            }
+            for (int c0 = 0; c0 < n; c0 += kBigLds) {
+                const int cnum = n - c0 < kBigLds ? n - c0 : kBigLds;
+                __syncthreads();
+                for (int i = tid; i < cnum; i += kBigThreads) sK[i] = fix_key(keys, tie, rd[a + c0 + i]);
+                __syncthreads();
+                for (int j = 0; j < cnum; ++j) {
+                    const uint64_t kj = sK[j];
+#pragma unroll
+                    for (int m = 0; m < kBigPer; ++m) rk[m] += kj < mk[m] ? 1 : 0;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < kBigPer; ++m)
+                if (t0 + tid + m * kBigThreads < n) put(a + rk[m], mv[m]);
+        }
+    }
 }
 
 // Slot j of the current order, stored with layout `from`, moved to its slot
@@ -335,9 +448,16 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
                            sb.counts, first, in_cell_order ? 0 : cpc);
         SMI_LAUNCH_CHECK();
         if (in_cell_order) {
+            if (sb.big) launch_fill_i32(sb.nbig, 2, 0, st);
             hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie);
+                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie, sb.big, sb.nbig);
             SMI_LAUNCH_CHECK();
+            if (sb.big) {
+                hipLaunchKernelGGL(k_bucket_fix_big, dim3(kBigBlocks), dim3(kBigThreads), 0, st, keys,
+                                   (const int32_t *)sb.cell_off, first, sb.perm, sb.tmp, cpc, tie,
+                                   (const int32_t *)sb.big, sb.nbig);
+                SMI_LAUNCH_CHECK();
+            }
         }
     }
 }

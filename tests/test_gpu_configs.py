@@ -133,3 +133,31 @@ def test_order4_mixed_vs_fp64_1000_steps():
     print("order-4 mixed vs fp64 after 1000 steps:", err)
     assert err["field"] < 1e-5 and err["kinetic"] < 1e-5 and err["total"] < 1e-5, err
     assert err["gauss"] < 1e-6, err
+
+
+def test_beam_plasma_mixed_vs_fp64_1000_steps():
+    """The beam-plasma configuration on a reduced 128x64x64 box (same dx, dt,
+    32x16x16 patches, e-/p+ background 8 ppc each at sigma 0.02, cold e-
+    beam gamma = 50, n = 0.1, 8 ppc within 16 cells of the axis: 9.2e6
+    particles), 1000 steps in both forms from identical initial particles
+    (the mixed form's FP32 state): energy errors <= 1e-5 of the total,
+    Gauss drift <= 1e-6."""
+    shape, patch, dx = (128, 64, 64), (32, 16, 16), (0.25, 0.25, 0.25)
+    ub = float(np.sqrt(50.0 ** 2 - 1.0))
+    parts = [("electron", 1.0, -1.0, thermal(shape, dx, 8, 0.02, 61)),
+             ("proton", 1836.0, 1.0, thermal(shape, dx, 8, 0.02 / np.sqrt(1836.0), 62)),
+             ("beam", 1.0, -1.0, thermal(shape, dx, 8, 0.0, 63, density=0.1, drift_ux=ub, radius=16.0))]
+    m = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=sa.MIXED)
+    f = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=sa.PLAIN_FP64)
+    for name, mass, q, p in parts:
+        load(m, m.add_species(name, mass, q, len(p["x"])), p)
+        # identical initial conditions: the FP64 run starts from the mixed
+        # form's own (FP32-rounded) particles
+        load(f, f.add_species(name, mass, q, len(p["x"])), m.get_particles(m.species.index(name)))
+    gm0, rho0 = m.gauss(), np.abs(m.rho()).max()
+    f.step(1000)
+    m.step(1000)
+    err = errors(m, f, gm0, rho0)
+    print("beam-plasma mixed vs fp64 after 1000 steps:", err)
+    assert err["field"] < 1e-5 and err["kinetic"] < 1e-5 and err["total"] < 1e-5, err
+    assert err["gauss"] < 1e-6, err

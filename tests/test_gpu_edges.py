@@ -1,7 +1,7 @@
 """Edge cases of the engine: empty species, the capacity limit, very dense
-cells (the in-cell fix-up's insertion sort, the mixed kernel's node-changer
-queue overflowing into the inline deposit), particles exactly on cell faces
-and at the end of the box."""
+cells (the workgroup sort of the in-cell fix-up, the mixed kernel's
+node-changer queue overflowing into the inline deposit), particles exactly
+on cell faces and at the end of the box."""
 import numpy as np
 import pytest
 
@@ -70,11 +70,14 @@ def dense_plasma(seed=5, n_dense=3000, sigma=0.3):
     return p
 
 
-def test_dense_cell_deterministic_bit_exact():
-    """3000 particles in one cell: buckets far beyond the register ranking
-    (insertion-sort fix-up) and a long per-patch deposit list; the
-    deterministic engine stays bit-identical to the oracle."""
-    p = dense_plasma()
+@pytest.mark.parametrize("n_dense", [3000, 5000])
+def test_dense_cell_deterministic_bit_exact(n_dense):
+    """3000 (5000) particles in one cell: a bucket far beyond the register
+    ranking, ordered by one workgroup with a bitonic sort in LDS (by rank
+    counting over LDS-staged chunks beyond 4096), and a long per-patch
+    deposit list; the deterministic engine stays bit-identical to the
+    oracle."""
+    p = dense_plasma(n_dense=n_dense)
     g = new(deposit=sa.DEPOSIT_DETERMINISTIC)
     o = O.OracleSim(3, 2, SHAPE, PATCH, DX, cfl_dt(DX))
     load(g, g.add_species("e", 1.0, -1.0, len(p["x"])), p)
