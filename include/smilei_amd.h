@@ -181,7 +181,7 @@ void *smi_stream(smi_context *ctx);
  * current state; smi_get_particles returns cell-major order either way.
  * SMI_TUNE_THREADS: workgroup size per 4^3
  * cluster, 64 | 128 | 256 | 512 (particles per thread = cluster particles /
- * threads).  SMI_TUNE_PERSISTENT (mixed form): 0 = one workgroup per
+ * threads).  SMI_TUNE_PERSISTENT (both forms): 0 = one workgroup per
  * cluster of the grid, 1 = a grid of the resident workgroups pulling the
  * non-empty clusters from a work list built each step. */
 #define SMI_TUNE_INTERLEAVE 1

@@ -94,7 +94,7 @@ struct smi_context {
     bool mixed = false;            // SMI_MIXED form
     int interleave = 1;            // lane-interleaved cluster layout of the sort (tuning)
     int threads = 256;             // workgroup size of the particle kernel (tuning)
-    int persistent = 0;            // persistent work-list form of the mixed kernel (tuning)
+    int persistent = 0;            // persistent work-list form of the particle kernel (tuning)
     int cell_order = 1;            // canonical order inside a cell (atomic deposit may drop it)
     int xcd = 0;                   // XCD-aware cluster order of the grid launch (tuning)
     int persistent_blocks = 0;     // its grid, from the occupancy of the current knobs
@@ -203,11 +203,13 @@ void upload_mix(smi_context *c)
     SMI_HIP(hipStreamSynchronize(c->st));  // v is released on return
 }
 
-// Grid of the persistent mixed kernel for the current knobs (queried outside
-// any graph capture).
+// Grid of the persistent particle kernel for the current knobs (queried
+// outside any graph capture).
 void refresh_persistent(smi_context *c)
 {
-    c->persistent_blocks = c->mixed && c->persistent ? mixed_persistent_blocks(c->g, c->threads) : 0;
+    if (!c->persistent) c->persistent_blocks = 0;
+    else if (c->mixed) c->persistent_blocks = mixed_persistent_blocks(c->g, c->threads);
+    else c->persistent_blocks = fp_persistent_blocks(c->g, c->p.deposit != SMI_DEPOSIT_DETERMINISTIC, c->threads);
 }
 
 // The particle kernel launches of one step reading buffer a, writing 1 - a.
@@ -233,7 +235,8 @@ void enqueue_particles(smi_context *c, int a)
         const int ns = (int)c->sp.size();
         for (int g0 = 0; g0 < ns; g0 += kMaxMerged)
             launch_particles(g, c->d_fp + a * ns + g0, std::min(kMaxMerged, ns - g0), GF, J, !det, c->err,
-                             c->threads, c->xcd != 0, c->st);
+                             c->threads, c->xcd != 0, c->persistent ? c->persistent_blocks : 0, c->d_work,
+                             c->d_work + g.npatches * g.clpp, c->st);
     }
 }
 
@@ -531,8 +534,8 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     if (c->mixed) {
         c->d_mix = dalloc<MixSpecies>(2 * c->sp.size());
         upload_mix(c);
-        if (!c->d_work) c->d_work = dalloc<int>((int64_t)c->g.npatches * c->g.clpp + 2);
     }
+    if (!c->d_work) c->d_work = dalloc<int>((int64_t)c->g.npatches * c->g.clpp + 2);
     SMI_HIP(hipStreamSynchronize(c->st));
     if (id) *id = (int)c->sp.size() - 1;
     return SMI_OK;
@@ -1095,7 +1098,6 @@ int smi_set_tuning(smi_context *c, int key, int value)
     } else if (key == SMI_TUNE_XCD) {
         c->xcd = value != 0;
     } else if (key == SMI_TUNE_PERSISTENT) {
-        if (value && !c->mixed) throw Error(SMI_ERR_INVALID, "SMI_TUNE_PERSISTENT applies to the mixed form");
         c->persistent = value != 0;
     } else {
         throw Error(SMI_ERR_INVALID, "unknown tuning key");
@@ -1131,8 +1133,8 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     SMI_HIP(hipSetDevice(c->p.device));
     SMI_HIP(hipStreamSynchronize(c->st));
     drop_graphs(c);
-    // candidates: cluster layout (tile-deposit forms) x persistent form
-    // (mixed) x cluster order x workgroup size; each timed on the live
+    // candidates: cluster layout (tile-deposit forms) x persistent form x
+    // cluster order x workgroup size; each timed on the live
     // particle state with the particle launches of one step.  The trial
     // launches only write the output ping-pong buffer (rewritten by the next
     // step), the destination histograms (cleared below) and, in the atomic
@@ -1140,7 +1142,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     // after re-sorting the current state into it.
     const int maps[2] = {1, 0};
     const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
-    const int nmap = det ? 1 : 2, npers = c->mixed ? 2 : 1;
+    const int nmap = det ? 1 : 2, npers = 2;
     const int sizes[3] = {128, 256, 512};
     int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent, best_xcd = c->xcd;
     float best = 3.4e38f;

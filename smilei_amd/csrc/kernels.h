@@ -61,8 +61,14 @@ struct FpSpecies {
 // outside its cluster's stencil tile (corrupt input) is left unchanged and
 // raises *err.  The kernel also accumulates the histogram of destination
 // cells into `counts` (zero on entry) for launch_sort.
+// persistent_blocks > 0 selects the persistent form: that many workgroups
+// pull the non-empty clusters from a work list (work: one int per cluster,
+// wcount: 2 ints), as for the mixed form.
 void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const GatherFields &F, const JPtrs &J,
-                      bool deposit, int *err, int threads, bool xcd, hipStream_t st);
+                      bool deposit, int *err, int threads, bool xcd, int persistent_blocks, int *work, int *wcount,
+                      hipStream_t st);
+// Resident workgroups of the persistent FP64 kernel on the current device.
+int fp_persistent_blocks(const Geom &g, bool deposit, int threads);
 // Largest workgroup size the particle kernel of this form accepts.
 int particle_max_threads(const Geom &g, bool mixed);
 

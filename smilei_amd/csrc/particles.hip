@@ -57,13 +57,13 @@ __device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, c
 // ---------------------------------------------------------------------------
 // Fused particle kernel.
 // ---------------------------------------------------------------------------
-// One workgroup per cluster runs the particles of every species of the
-// launch (specs[0..ns)) against one field tile (and, with DEPOSIT, one J
-// tile).
+// One workgroup runs the particles of every species of the launch
+// (specs[0..ns)) of cluster `cluster` against one field tile (and, with
+// DEPOSIT, one J tile).  Uniform across the workgroup; returns at once for
+// an empty cluster.
 template <int ND, int ORDER, bool DEPOSIT>
-__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
-k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err,
-            int xcd)
+__device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ specs, int ns, const GatherFields &F,
+                                  const JPtrs &J, int *__restrict__ err, int cluster)
 {
     using TL = Tile<ORDER>;
     constexpr int H = TL::H, GT = TL::GT, JT = TL::JT, S = TL::S;
@@ -84,7 +84,7 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
     __shared__ double sQ[QCAP][8];
     __shared__ int sQn;
 
-    const Cluster cl = decode_cluster(g, xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x);
+    const Cluster cl = decode_cluster(g, cluster);
     int total = 0;
     for (int q = 0; q < ns; ++q)
         total += specs[q].cell_off[cl.gbase + g.cells_per_cl] - specs[q].cell_off[cl.gbase];
@@ -124,9 +124,9 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
         pmax[d] = (double)(cl.pc[d] + 1) * g.plen[d];
     }
 
-    // Particles of the cluster are sorted by cell (about ppc per cell).  Lane l
-    // of every wavefront takes slot l*q + row, so the 64 lanes of a wave work
-    // in different cells and their LDS atomics rarely hit the same address.
+    // A wavefront takes 64 consecutive slots of the cluster's range; stored
+    // lane-interleaved by the sort, they hold particles of different cells,
+    // so the lanes' LDS atomics rarely hit the same address.
     for (int sid = 0; sid < ns; ++sid) {
     const FpSpecies &Sp = specs[sid];
     const SpeciesK &sp = Sp.k;
@@ -346,6 +346,56 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
     }
 }
 
+// One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
+template <int ND, int ORDER, bool DEPOSIT>
+__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
+k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err,
+            int xcd)
+{
+    fp_cluster<ND, ORDER, DEPOSIT>(g, specs, ns, F, J, err, xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
+// Persistent form: a grid sized to the resident workgroups pulls clusters
+// from the work list (the non-empty clusters, wcount[0] of them) through the
+// queue head wcount[1]; every workgroup leaves once the list is exhausted.
+template <int ND, int ORDER, bool DEPOSIT>
+__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
+k_particles_list(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err,
+                 const int *__restrict__ work, int *__restrict__ wcount)
+{
+    __shared__ int sItem;
+    const int nwork = wcount[0];
+    for (;;) {
+        __syncthreads();  // the previous cluster is done with the LDS tiles
+        if (threadIdx.x == 0) sItem = atomicAdd(&wcount[1], 1);
+        __syncthreads();
+        const int item = sItem;
+        if (item >= nwork) break;
+        fp_cluster<ND, ORDER, DEPOSIT>(g, specs, ns, F, J, err, work[item]);
+    }
+}
+
+// Work list of the clusters holding particles of any species of the launch
+// (order irrelevant); wcount[0..1] must be zero.
+__global__ void k_fp_worklist(Geom g, const FpSpecies *__restrict__ specs, int ns, int *__restrict__ work,
+                              int *__restrict__ wcount)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = 0;
+    if (c < g.npatches * g.clpp) {
+        const int64_t gbase = (int64_t)(c / g.clpp) * g.cpp + (int64_t)(c % g.clpp) * g.cells_per_cl;
+        for (int q = 0; q < ns; ++q) total += specs[q].cell_off[gbase + g.cells_per_cl] - specs[q].cell_off[gbase];
+    }
+    const uint64_t m = __ballot(total > 0);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&wcount[0], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (total > 0) work[base + __popcll(m & ((1ull << lane) - 1))] = c;
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic deposit (reference order) into per-patch ghosted J.
 // ---------------------------------------------------------------------------
@@ -562,7 +612,8 @@ __global__ void k_fold_interior(Geom g, const double *__restrict__ pj, JPtrs J)
 int64_t patch_j_size(const Geom &g) { return pj_layout(g).per; }
 
 void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const GatherFields &F, const JPtrs &J,
-                      bool deposit, int *err, int threads, bool xcd, hipStream_t st)
+                      bool deposit, int *err, int threads, bool xcd, int persistent_blocks, int *work, int *wcount,
+                      hipStream_t st)
 {
     if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
         throw Error(-1, "particle-kernel workgroup size must be 64, 128, 256 or 512");
@@ -570,9 +621,22 @@ void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const Gathe
         throw Error(-1, "workgroup size above the particle kernel's limit for this geometry and order");
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
     const int nb = g.npatches * g.clpp;
-    const dim3 grid(nb), block(threads);
-#define SMI_PK(ND, O, D) hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, specs, ns, F, J, err, \
-                                            xcd ? 1 : 0)
+    const dim3 block(threads);
+    if (persistent_blocks > 0) {
+        // the work list, then as many workgroups as fit at once
+        launch_fill_i32(wcount, 2, 0, st);
+        hipLaunchKernelGGL(k_fp_worklist, dim3((nb + 255) / 256), dim3(256), 0, st, g, specs, ns, work, wcount);
+        SMI_LAUNCH_CHECK();
+    }
+    const dim3 grid(persistent_blocks > 0 ? std::min(nb, persistent_blocks) : nb);
+#define SMI_PK(ND, O, D)                                                                                          \
+    do {                                                                                                          \
+        if (persistent_blocks > 0)                                                                                \
+            hipLaunchKernelGGL((k_particles_list<ND, O, D>), grid, block, 0, st, g, specs, ns, F, J, err,         \
+                               (const int *)work, wcount);                                                         \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_particles<ND, O, D>), grid, block, 0, st, g, specs, ns, F, J, err, xcd ? 1 : 0); \
+    } while (0)
     if (g.nd == 3 && g.order == 2) {
         if (deposit) SMI_PK(3, 2, true);
         else SMI_PK(3, 2, false);
@@ -587,6 +651,22 @@ void launch_particles(const Geom &g, const FpSpecies *specs, int ns, const Gathe
     }
 #undef SMI_PK
     SMI_LAUNCH_CHECK();
+}
+
+int fp_persistent_blocks(const Geom &g, bool deposit, int threads)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    SMI_HIP(hipGetDevice(&dev));
+    SMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const void *fn = nullptr;
+    if (g.nd == 3 && g.order == 2)
+        fn = deposit ? (const void *)k_particles_list<3, 2, true> : (const void *)k_particles_list<3, 2, false>;
+    else if (g.nd == 3 && g.order == 4)
+        fn = deposit ? (const void *)k_particles_list<3, 4, true> : (const void *)k_particles_list<3, 4, false>;
+    else
+        fn = deposit ? (const void *)k_particles_list<2, 2, true> : (const void *)k_particles_list<2, 2, false>;
+    SMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
+    return std::max(1, cus * std::max(per_cu, 1));
 }
 
 int particle_max_threads(const Geom &g, bool mixed)

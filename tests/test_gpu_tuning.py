@@ -44,7 +44,7 @@ def test_autotune_keeps_state(precision):
     b.step(2)
     knobs = b.autotune(reps=2)
     assert knobs["threads"] in (128, 256, 512) and knobs["interleave"] in (0, 1)
-    assert knobs["persistent"] in ((0, 1) if precision == sa.MIXED else (0,))
+    assert knobs["persistent"] in (0, 1)
     assert knobs["xcd"] in (0, 1) and not (knobs["xcd"] and knobs["persistent"])
     assert knobs["particles_ms"] > 0
     a.step(5)
@@ -65,23 +65,22 @@ def test_tuning_rejects_bad_values():
         sim.set_tuning(sa.TUNE_THREADS, 96)
     with pytest.raises(RuntimeError, match="unknown tuning key"):
         sim.set_tuning(7, 1)
-    fp64 = make(sa.PLAIN_FP64, shape=(16, 16, 16))
-    with pytest.raises(RuntimeError, match="mixed form"):
-        fp64.set_tuning(sa.TUNE_PERSISTENT, 1)
 
 
+@pytest.mark.parametrize("precision", [sa.MIXED, sa.PLAIN_FP64])
 @pytest.mark.parametrize("uneven", [False, True])
-def test_persistent_work_list_matches_grid_launch(uneven):
-    """The persistent work-list launch visits exactly the non-empty clusters:
-    same keys and counts, fields equal up to the order of the J flush."""
+def test_persistent_work_list_matches_grid_launch(uneven, precision):
+    """The persistent work-list launch (both forms) visits exactly the
+    non-empty clusters: same keys and counts, fields equal up to the order of
+    the J flush."""
     from tests.plasma import lwfa
     if uneven:
         shape, patch, dx = (256, 32, 32), (32, 16, 16), (0.1, 0.5, 0.5)
-        a = lwfa(sa.Simulation, sa.MIXED, shape, patch, dx)
-        b = lwfa(sa.Simulation, sa.MIXED, shape, patch, dx)
+        a = lwfa(sa.Simulation, precision, shape, patch, dx)
+        b = lwfa(sa.Simulation, precision, shape, patch, dx)
         ns = 1
     else:
-        a, b, ns = make(sa.MIXED), make(sa.MIXED), 2
+        a, b, ns = make(precision), make(precision), 2
     b.set_tuning(sa.TUNE_PERSISTENT, 1)
     assert b.get_tuning(sa.TUNE_PERSISTENT) == 1
     a.step(6)
@@ -131,3 +130,20 @@ def test_xcd_order_mixed_matches_grid_order():
     for f in ("Ex", "Jz"):
         ref = a.get_field(f)
         assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
+
+
+def test_persistent_deterministic_fp64_is_bit_identical():
+    """The deterministic FP64 form run by the persistent work-list kernel:
+    particles, keys and every field bit-identical to the grid launch."""
+    from tests.plasma import lwfa
+    shape, patch, dx = (256, 32, 32), (32, 16, 16), (0.1, 0.5, 0.5)
+    a = lwfa(sa.Simulation, sa.PLAIN_FP64, shape, patch, dx, deposit=sa.DEPOSIT_DETERMINISTIC)
+    b = lwfa(sa.Simulation, sa.PLAIN_FP64, shape, patch, dx, deposit=sa.DEPOSIT_DETERMINISTIC)
+    b.set_tuning(sa.TUNE_PERSISTENT, 1)
+    a.step(5)
+    b.step(5)
+    pa, pb = a.get_particles(0), b.get_particles(0)
+    for k in ("x", "y", "z", "px", "py", "pz", "w", "key", "patch"):
+        np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
+    for f in ("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"):
+        np.testing.assert_array_equal(a.get_field(f), b.get_field(f), err_msg=f)
