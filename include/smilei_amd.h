@@ -191,6 +191,10 @@ void *smi_stream(smi_context *ctx);
  * each; workgroups are dispatched to them round-robin) a contiguous run of
  * clusters, so that neighbouring clusters share L2 lines; 0 = grid order. */
 #define SMI_TUNE_XCD 5
+/* SMI_TUNE_TILE (mixed form): LDS tile shape of the particle kernel, 1 = J
+ * rows padded by one 64-bit word (fewer bank conflicts of the tile atomics
+ * on dense plasmas), 0 = unpadded. */
+#define SMI_TUNE_TILE 6
 /* Not searched by the autotuner (it changes what smi_get_particles returns):
  * SMI_TUNE_CELL_ORDER 1 (default) keeps the canonical order of particles
  * inside a cell (direction of motion, then previous slot); 0 leaves it to

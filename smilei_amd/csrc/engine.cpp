@@ -97,6 +97,7 @@ struct smi_context {
     int persistent = 0;            // persistent work-list form of the particle kernel (tuning)
     int cell_order = 1;            // canonical order inside a cell (atomic deposit may drop it)
     int xcd = 0;                   // XCD-aware cluster order of the grid launch (tuning)
+    int tile = 1;                  // J-tile shape of the mixed kernel: rows padded by one word (tuning)
     int persistent_blocks = 0;     // its grid, from the occupancy of the current knobs
     int *d_work = nullptr;         // work list (one int per cluster) + head/count (2 ints)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -208,7 +209,7 @@ void upload_mix(smi_context *c)
 void refresh_persistent(smi_context *c)
 {
     if (!c->persistent) c->persistent_blocks = 0;
-    else if (c->mixed) c->persistent_blocks = mixed_persistent_blocks(c->g, c->threads);
+    else if (c->mixed) c->persistent_blocks = mixed_persistent_blocks(c->g, c->threads, c->tile);
     else c->persistent_blocks = fp_persistent_blocks(c->g, c->p.deposit != SMI_DEPOSIT_DETERMINISTIC, c->threads);
 }
 
@@ -227,7 +228,7 @@ void enqueue_particles(smi_context *c, int a)
             const int n = std::min(kMaxMerged, ns - g0);
             double cr_max = 0.;
             for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
-            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->threads, c->xcd != 0,
+            launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->threads, c->xcd != 0, c->tile,
                                    c->persistent ? c->persistent_blocks : 0, c->d_work,
                                    c->d_work + g.npatches * g.clpp, c->st);
         }
@@ -1099,6 +1100,9 @@ int smi_set_tuning(smi_context *c, int key, int value)
         c->xcd = value != 0;
     } else if (key == SMI_TUNE_PERSISTENT) {
         c->persistent = value != 0;
+    } else if (key == SMI_TUNE_TILE) {
+        if (value != 0 && value != 1) throw Error(SMI_ERR_INVALID, "SMI_TUNE_TILE must be 0 or 1");
+        c->tile = value;
     } else {
         throw Error(SMI_ERR_INVALID, "unknown tuning key");
     }
@@ -1119,6 +1123,7 @@ int smi_get_tuning(smi_context *c, int key, int *value)
     else if (key == SMI_TUNE_THREADS) *value = c->threads;
     else if (key == SMI_TUNE_PERSISTENT) *value = c->persistent;
     else if (key == SMI_TUNE_XCD) *value = c->xcd;
+    else if (key == SMI_TUNE_TILE) *value = c->tile;
     else if (key == SMI_TUNE_CELL_ORDER) *value = c->cell_order || c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
     else throw Error(SMI_ERR_INVALID, "unknown tuning key");
     return SMI_OK;
@@ -1133,8 +1138,8 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     SMI_HIP(hipSetDevice(c->p.device));
     SMI_HIP(hipStreamSynchronize(c->st));
     drop_graphs(c);
-    // candidates: cluster layout (tile-deposit forms) x persistent form x
-    // cluster order x workgroup size; each timed on the live
+    // candidates: cluster layout (tile-deposit forms) x J-tile shape (mixed)
+    // x persistent form x cluster order x workgroup size; each timed on the live
     // particle state with the particle launches of one step.  The trial
     // launches only write the output ping-pong buffer (rewritten by the next
     // step), the destination histograms (cleared below) and, in the atomic
@@ -1142,15 +1147,17 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     // after re-sorting the current state into it.
     const int maps[2] = {1, 0};
     const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
-    const int nmap = det ? 1 : 2, npers = 2;
+    const int nmap = det ? 1 : 2, npers = 2, ntile = c->mixed ? 2 : 1;
     const int sizes[3] = {128, 256, 512};
     int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent, best_xcd = c->xcd;
+    int best_tile = c->tile;
     float best = 3.4e38f;
     for (int m = 0; m < nmap; ++m) {
     if (!det && c->interleave != maps[m]) {
         c->interleave = maps[m];
         apply_layout(c);
     }
+    for (int tl = 0; tl < ntile; ++tl)
     for (int pm = 0; pm < npers; ++pm)
     for (int xo = 0; xo < 2; ++xo)
         for (int z = 0; z < 3; ++z) {
@@ -1159,6 +1166,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
             c->threads = sizes[z];
             c->persistent = pm;
             c->xcd = xo;
+            if (c->mixed) c->tile = 1 - tl;
             refresh_persistent(c);
             enqueue_particles(c, c->cur);  // warm-up
             SMI_HIP(hipEventRecord(c->ev[0], c->st));
@@ -1174,6 +1182,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
                 best_threads = c->threads;
                 best_pers = c->persistent;
                 best_xcd = c->xcd;
+                best_tile = c->tile;
             }
         }
     }
@@ -1181,6 +1190,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     c->threads = best_threads;
     c->persistent = best_pers;
     c->xcd = best_xcd;
+    c->tile = best_tile;
     refresh_persistent(c);
     apply_layout(c);
     for (auto &s : c->sp) launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);

@@ -174,10 +174,10 @@ struct MixSpecies {
 // selects the persistent form: that many workgroups pull the non-empty
 // clusters from a work list (work: one int per cluster, wcount: 2 ints).
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, int threads, bool xcd, int persistent_blocks, int *work,
-                            int *wcount, hipStream_t st);
+                            const JPtrs &J, int *err, int threads, bool xcd, int tile, int persistent_blocks,
+                            int *work, int *wcount, hipStream_t st);
 // Resident workgroups of the persistent mixed kernel on the current device.
-int mixed_persistent_blocks(const Geom &g, int threads);
+int mixed_persistent_blocks(const Geom &g, int threads, int tile);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
 // FP64 global positions of the mixed particles in buffer order, and the
 // key (direction 13) of the cell each lies in.

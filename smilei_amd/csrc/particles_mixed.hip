@@ -221,16 +221,17 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
 // floats (not 8*8 = 64, a multiple of the LDS bank count) spreads the four
 // x-planes of cells that the lanes of a wavefront cover over different
 // banks (measured on the order-2 kernel: x-strides of 68, 76, 84 equal,
-// 72 and 80 6 % slower).  The node-changer queue takes the LDS left in the budget that
-// keeps four workgroups per CU (order 2).
-template <int ORDER>
+// 72 and 80 6 % slower).  JPAD = 1 pads the J rows by one word (a tile
+// shape chosen by the autotuner: on the beam-plasma the padded rows meet
+// fewer bank conflicts of the 64-bit atomics, 3 % faster).  The node-changer
+// queue takes the LDS left in the budget that keeps four workgroups per CU
+// (order 2).
+template <int ORDER, int JPAD>
 struct MixTile {
     static constexpr int H = ORDER / 2;
     static constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
     static constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GT * GSX;
-    // order 2: J rows padded to 10 words (a y-stride of 9 = JT, x-stride 81
-    // collided more often in the bank of the 64-bit atomics: 3 % slower)
-    static constexpr int JSY = ORDER == 2 ? JT + 1 : JT, JSX = JT * JSY, JSC = JT * JSX;
+    static constexpr int JSY = JT + JPAD, JSX = JT * JSY, JSC = JT * JSX;
     static constexpr int kBudget = ORDER == 2 ? 40960 - 64 : 65536;
     static constexpr int kFixed = 6 * GF * 4 + 3 * JSC * 8 + kMaxMerged * 64 * 4 + 16;
     static constexpr int QMAX = ORDER == 2 ? 256 : 128;
@@ -242,11 +243,11 @@ struct MixTile {
 // Esirkepov deposit of one particle from its old and new in-cell deltas
 // (dold, dnew in [-1/2, 1/2)), node shifts sh and the tile coordinates o3 of
 // its stencil origin; NT/NA as esirkepov3_fixed.
-template <int ORDER, int NT, int NA>
+template <int ORDER, int NT, int NA, int JPAD>
 __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, const float *dnew, const int *sh,
                                      const int *o3, const double *crq)
 {
-    using TL = MixTile<ORDER>;
+    using TL = MixTile<ORDER, JPAD>;
     double S0[3][ORDER + 2], DS[3][ORDER + 2];
     int w0[3];
     if constexpr (NT == ORDER + 1) {
@@ -304,11 +305,11 @@ __device__ inline Cl cluster_of(const Geom &g, int b)
 // cluster's range of perm: a wavefront reads 64 consecutive slots, whose
 // particles lie in different cells when the sort stored the range in the
 // lane-interleaved layout (cluster_slot).
-template <int ORDER>
+template <int ORDER, int JPAD>
 __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict__ specs, int ns, double cr_max,
                                      const GatherFields &F, const JPtrs &J, int *__restrict__ err, int cluster)
 {
-    using TL = MixTile<ORDER>;
+    using TL = MixTile<ORDER, JPAD>;
     constexpr int H = TL::H, GO = TL::GO, GT = TL::GT, JO = TL::JO, JT = TL::JT;
     constexpr int GSX = TL::GSX, GSY = TL::GSY, GF = TL::GF;
     constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
@@ -480,7 +481,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             // contributions in units of the cluster's fixed-point quantum
             const double cwq = sp.cwd * (double)w * inv_quantum;
             const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
-            if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER>(sJ, dold, dnew, sh, o3, crq);
+            if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER, JPAD>(sJ, dold, dnew, sh, o3, crq);
             // node changers: one queue slot range per wavefront
             const uint64_t gm = __ballot(moved != 0);
             if (gm) {
@@ -503,7 +504,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
                         sQ[pos][6] = w;
                         sQ[pos][7] = __int_as_float(packed);
                     } else {
-                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1>(sJ, dold, dnew, sh, o3, crq);
+                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD>(sJ, dold, dnew, sh, o3, crq);
                     }
                 }
             }
@@ -545,7 +546,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const int packed = __float_as_int(sQ[t][7]);
         const int o3[3] = {packed & 7, (packed >> 3) & 7, (packed >> 6) & 7};
         const int sh[3] = {((packed >> 9) & 3) - 1, ((packed >> 11) & 3) - 1, ((packed >> 13) & 3) - 1};
-        deposit_fixed<ORDER, ORDER + 2, ORDER + 1>(sJ, dold, dnew, sh, o3, crq);
+        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD>(sJ, dold, dnew, sh, o3, crq);
     }
     __syncthreads();
     double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
@@ -564,19 +565,19 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 // One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
 // Order 2: at most 128 VGPRs and 40 KB of LDS, so that four wavefronts per
 // SIMD are resident (four 256-thread workgroups per CU).
-template <int ORDER>
+template <int ORDER, int JPAD>
 __global__ void __launch_bounds__(512, ORDER == 2 ? 4 : 2)
 k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
                   int *__restrict__ err, int xcd)
 {
     const int b = xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x;
-    mixed_cluster<ORDER>(g, specs, ns, cr_max, F, J, err, b);
+    mixed_cluster<ORDER, JPAD>(g, specs, ns, cr_max, F, J, err, b);
 }
 
 // Persistent form: a grid sized to the resident workgroups pulls clusters
 // from the work list (the non-empty clusters, wcount[0] of them) through the
 // queue head wcount[1]; every workgroup leaves once the list is exhausted.
-template <int ORDER>
+template <int ORDER, int JPAD>
 __global__ void __launch_bounds__(512, ORDER == 2 ? 3 : 2)
 k_particles_mixed_list(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
                        int *__restrict__ err, const int *__restrict__ work, int *__restrict__ wcount)
@@ -589,7 +590,7 @@ k_particles_mixed_list(Geom g, const MixSpecies *__restrict__ specs, int ns, dou
         __syncthreads();
         const int item = sItem;
         if (item >= nwork) break;
-        mixed_cluster<ORDER>(g, specs, ns, cr_max, F, J, err, work[item]);
+        mixed_cluster<ORDER, JPAD>(g, specs, ns, cr_max, F, J, err, work[item]);
     }
 }
 
@@ -747,9 +748,10 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }  // namespace
 
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, int threads, bool xcd, int persistent_blocks, int *work,
-                            int *wcount, hipStream_t st)
+                            const JPtrs &J, int *err, int threads, bool xcd, int tile, int persistent_blocks,
+                            int *work, int *wcount, hipStream_t st)
 {
+    if (tile != 0 && tile != 1) throw Error(-1, "mixed tile shape must be 0 or 1");
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
     if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
@@ -757,10 +759,15 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
     const int nclusters = g.npatches * g.clpp;
     const dim3 block(threads);
     if (persistent_blocks <= 0) {
-#define SMI_PM(O) hipLaunchKernelGGL((k_particles_mixed<O>), dim3(nclusters), block, 0, st, g, specs, ns, cr_max, \
-                                     F, J, err, xcd ? 1 : 0)
-        if (g.order == 2) SMI_PM(2);
-        else SMI_PM(4);
+#define SMI_PM(O, P) hipLaunchKernelGGL((k_particles_mixed<O, P>), dim3(nclusters), block, 0, st, g, specs, ns, \
+                                        cr_max, F, J, err, xcd ? 1 : 0)
+        if (g.order == 2) {
+            if (tile) SMI_PM(2, 1);
+            else SMI_PM(2, 0);
+        } else {
+            if (tile) SMI_PM(4, 1);
+            else SMI_PM(4, 0);
+        }
 #undef SMI_PM
         SMI_LAUNCH_CHECK();
         return;
@@ -770,20 +777,27 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
     hipLaunchKernelGGL(k_worklist, dim3((nclusters + 255) / 256), dim3(256), 0, st, g, specs, ns, work, wcount);
     SMI_LAUNCH_CHECK();
     const int nblocks = std::min(nclusters, persistent_blocks);
-#define SMI_PL(O) hipLaunchKernelGGL((k_particles_mixed_list<O>), dim3(nblocks), block, 0, st, g, specs, ns, \
-                                     cr_max, F, J, err, (const int *)work, wcount)
-    if (g.order == 2) SMI_PL(2);
-    else SMI_PL(4);
+#define SMI_PL(O, P) hipLaunchKernelGGL((k_particles_mixed_list<O, P>), dim3(nblocks), block, 0, st, g, specs, \
+                                        ns, cr_max, F, J, err, (const int *)work, wcount)
+    if (g.order == 2) {
+        if (tile) SMI_PL(2, 1);
+        else SMI_PL(2, 0);
+    } else {
+        if (tile) SMI_PL(4, 1);
+        else SMI_PL(4, 0);
+    }
 #undef SMI_PL
     SMI_LAUNCH_CHECK();
 }
 
-int mixed_persistent_blocks(const Geom &g, int threads)
+int mixed_persistent_blocks(const Geom &g, int threads, int tile)
 {
     int dev = 0, cus = 0, per_cu = 0;
     SMI_HIP(hipGetDevice(&dev));
     SMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const void *fn = g.order == 2 ? (const void *)k_particles_mixed_list<2> : (const void *)k_particles_mixed_list<4>;
+    const void *fn = nullptr;
+    if (g.order == 2) fn = tile ? (const void *)k_particles_mixed_list<2, 1> : (const void *)k_particles_mixed_list<2, 0>;
+    else fn = tile ? (const void *)k_particles_mixed_list<4, 1> : (const void *)k_particles_mixed_list<4, 0>;
     SMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
     return std::max(1, cus * std::max(per_cu, 1));
 }

@@ -46,6 +46,7 @@ def test_autotune_keeps_state(precision):
     assert knobs["threads"] in (128, 256, 512) and knobs["interleave"] in (0, 1)
     assert knobs["persistent"] in (0, 1)
     assert knobs["xcd"] in (0, 1) and not (knobs["xcd"] and knobs["persistent"])
+    assert knobs["tile"] in (0, 1)
     assert knobs["particles_ms"] > 0
     a.step(5)
     b.step(5)
@@ -65,6 +66,8 @@ def test_tuning_rejects_bad_values():
         sim.set_tuning(sa.TUNE_THREADS, 96)
     with pytest.raises(RuntimeError, match="unknown tuning key"):
         sim.set_tuning(7, 1)
+    with pytest.raises(RuntimeError, match="SMI_TUNE_TILE"):
+        sim.set_tuning(sa.TUNE_TILE, 2)
 
 
 @pytest.mark.parametrize("precision", [sa.MIXED, sa.PLAIN_FP64])
@@ -147,3 +150,22 @@ def test_persistent_deterministic_fp64_is_bit_identical():
         np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
     for f in ("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"):
         np.testing.assert_array_equal(a.get_field(f), b.get_field(f), err_msg=f)
+
+
+def test_tile_shapes_agree():
+    """The two J-tile shapes of the mixed kernel: same keys and counts,
+    fields equal up to the order of the J flush."""
+    a, b = make(sa.MIXED), make(sa.MIXED)
+    a.set_tuning(sa.TUNE_TILE, 1)
+    b.set_tuning(sa.TUNE_TILE, 0)
+    assert (a.get_tuning(sa.TUNE_TILE), b.get_tuning(sa.TUNE_TILE)) == (1, 0)
+    a.step(6)
+    b.step(6)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+        np.testing.assert_allclose(pb["x"], pa["x"], rtol=1e-10)
+    for f in ("Ex", "Jz", "By"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
