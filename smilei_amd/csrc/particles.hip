@@ -130,13 +130,13 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
     for (int sid = 0; sid < ns; ++sid) {
     const FpSpecies &Sp = specs[sid];
     const SpeciesK &sp = Sp.k;
-    const PartPtrs A = Sp.A, B = Sp.B;
-    const int32_t *__restrict__ perm = Sp.perm;
-    uint32_t *__restrict__ keys = Sp.keys;
+    const GPart A = glb(Sp.A), B = glb(Sp.B);
+    const gptr<const int32_t> perm = glb(Sp.perm);
+    const gptr<uint32_t> keys = glb(Sp.keys);
     int32_t *__restrict__ counts = Sp.counts;
-    int32_t *__restrict__ rank = Sp.rank;
+    const gptr<int32_t> rank = glb(Sp.rank);
     const int lay = Sp.lay;
-    double *__restrict__ scratch = Sp.scratch;
+    const gptr<double> scratch = glb(Sp.scratch);
     const int64_t cap = Sp.cap;
     const int s = Sp.cell_off[cl.gbase], e = Sp.cell_off[cl.gbase + g.cells_per_cl];
     const int N = e - s;

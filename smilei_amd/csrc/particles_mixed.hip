@@ -359,12 +359,12 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     for (int sid = 0; sid < ns; ++sid) {
     const MixSpecies &S = specs[sid];
     const SpeciesKM &sp = S.k;
-    const MixPtrs A = S.A, B = S.B;
-    const uint32_t *__restrict__ keysA = S.keysA;
-    uint32_t *__restrict__ keysB = S.keysB;
-    const int32_t *__restrict__ perm = S.perm;
+    const GMix A = glb(S.A), B = glb(S.B);
+    const gptr<const uint32_t> keysA = glb(S.keysA);
+    const gptr<uint32_t> keysB = glb(S.keysB);
+    const gptr<const int32_t> perm = glb(S.perm);
     int32_t *__restrict__ counts = S.counts;
-    int32_t *__restrict__ rank = S.rank;
+    const gptr<int32_t> rank = glb(S.rank);
     const int lay = S.lay;
     const int s = S.cell_off[cl.gbase], e = S.cell_off[cl.gbase + 64];
     const int N = e - s;

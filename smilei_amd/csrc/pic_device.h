@@ -23,6 +23,34 @@ template <int ORDER> struct Tile {
     static constexpr int S = 2 * H + 3;      // Esirkepov stencil length
 };
 
+// Pointers into device memory, in the global address space.  The particle
+// arrays reach the kernels through descriptors in memory, which leaves the
+// compiler generic (flat) pointers: their loads and stores need a 64-bit
+// address in VGPRs per access and count against both the memory and the
+// LDS wait counters.  Cast to the global address space, they become global
+// loads and stores with a scalar base and a 32-bit offset.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ inline gptr<T> glb(T *p)
+{
+    return (gptr<T>)p;
+}
+struct GMix {
+    gptr<float> ox, oy, oz, ux, uy, uz, w;
+};
+__device__ inline GMix glb(const MixPtrs &M)
+{
+    return GMix{glb(M.ox), glb(M.oy), glb(M.oz), glb(M.ux), glb(M.uy), glb(M.uz), glb(M.w)};
+}
+struct GPart {
+    gptr<double> x, y, z, px, py, pz, w;
+};
+__device__ inline GPart glb(const PartPtrs &M)
+{
+    return GPart{glb(M.x), glb(M.y), glb(M.z), glb(M.px), glb(M.py), glb(M.pz), glb(M.w)};
+}
+
 // XCD-aware order of the one-workgroup-per-cluster grid.  Workgroups are
 // dispatched round-robin over the 8 XCDs of the chip (workgroup b to XCD
 // b % 8), each XCD with its own L2; this mapping gives XCD x the contiguous
