@@ -139,13 +139,20 @@ bool want_rank(const smi_context *c)
     return want_interleaved(c) && c->cell_order;
 }
 
+// The particle kernel places its cluster's stayers in the step's sort
+// (SortBufs::stay) when the order inside cells is the scatter's own.
+bool want_placed(const smi_context *c)
+{
+    return c->p.deposit != SMI_DEPOSIT_DETERMINISTIC && !c->cell_order;
+}
+
 // Sort of species S from keys; tie: the kernel's recorded positions (the
 // step's sort) or null (the slots themselves: freshly loaded particles).
 void sort_species(smi_context *c, SpeciesState &S, const uint32_t *keys, bool counted, bool ordered,
                   const int32_t *tie = nullptr)
 {
     S.lay = want_interleaved(c);
-    launch_sort(c->g, keys, S.n, S.sb, c->st, counted, ordered, S.lay, tie);
+    launch_sort(c->g, keys, S.n, S.sb, c->st, counted, ordered, S.lay, tie, counted && want_placed(c));
 }
 
 void drop_graphs(smi_context *c)
@@ -197,6 +204,7 @@ void upload_mix(smi_context *c)
             m.cell_off = s.sb.cell_off;
             m.counts = s.sb.counts;
             m.rank = want_rank(c) ? s.sb.rank : nullptr;
+            m.stay = want_placed(c) ? s.sb.stay : nullptr;
             m.lay = want_interleaved(c) ? 1 : 0;
             v.push_back(m);
         }
@@ -258,6 +266,7 @@ void upload_fp(smi_context *c)
             f.counts = s.sb.counts;
             f.scratch = s.scratch;
             f.rank = want_rank(c) ? s.sb.rank : nullptr;
+            f.stay = want_placed(c) ? s.sb.stay : nullptr;
             f.lay = want_interleaved(c) ? 1 : 0;
             f.cap = s.cap;
             v.push_back(f);
@@ -458,6 +467,7 @@ int smi_destroy(smi_context *c)
         (void)hipFree(s.sb.big);
         (void)hipFree(s.sb.nbig);
         (void)hipFree(s.sb.bsum);
+        (void)hipFree(s.sb.stay);
         (void)hipFree(s.scratch);
     }
     (void)hipFree(c->pj);
@@ -500,6 +510,8 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     if (c->p.deposit != SMI_DEPOSIT_DETERMINISTIC) s.sb.rank = dalloc<int32_t>(capacity);
     s.sb.nbsum = sort_bsum_size(c->g.ncells);
     s.sb.bsum = dalloc<int32_t>(s.sb.nbsum);
+    s.sb.stay = (uint8_t *)dalloc<int32_t>((c->g.ncells + 3) / 4);
+    launch_fill_i32((int32_t *)s.sb.stay, (c->g.ncells + 3) / 4, 0, c->st);
     if (c->p.deposit == SMI_DEPOSIT_DETERMINISTIC) s.scratch = dalloc<double>(7 * capacity);
     launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
     launch_fill_i32(s.sb.cell_off, c->g.ncells + 1, 0, c->st);
@@ -1142,7 +1154,8 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     // x persistent form x cluster order x workgroup size; each timed on the live
     // particle state with the particle launches of one step.  The trial
     // launches only write the output ping-pong buffer (rewritten by the next
-    // step), the destination histograms (cleared below) and, in the atomic
+    // step), the destination histograms and placed-stayer counts (cleared
+    // below) and, in the atomic
     // deposit, J (cleared at the start of every step).  A layout is tried
     // after re-sorting the current state into it.
     const int maps[2] = {1, 0};
@@ -1193,7 +1206,10 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     c->tile = best_tile;
     refresh_persistent(c);
     apply_layout(c);
-    for (auto &s : c->sp) launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
+    for (auto &s : c->sp) {
+        launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);
+        launch_fill_i32((int32_t *)s.sb.stay, (c->g.ncells + 3) / 4, 0, c->st);
+    }
     SMI_HIP(hipStreamSynchronize(c->st));
     if (best_ms) *best_ms = best;
     return SMI_OK;

@@ -48,6 +48,7 @@ struct FpSpecies {
     int32_t *counts;                 // destination histogram
     double *scratch;                 // 7*cap doubles (deterministic deposit)
     int32_t *rank;                   // or null: cell-major input position of each output slot
+    uint8_t *stay;                   // or null: placed stayers per cell (SortBufs::stay)
     int lay;                         // the input's clusters are stored lane-interleaved
     int64_t cap;
 };
@@ -129,7 +130,16 @@ struct SortBufs {
     int32_t *nbig;     // 2 (count, queue head of `big`)
     int32_t *bsum;     // scan scratch
     int64_t nbsum;
+    // Placed stayers: when the particle kernel is given `stay`, a particle
+    // whose destination cell lies in its own cluster gets its rank r among
+    // that cluster's stayers of the cell as the key's top five bits when
+    // r < kPlaceMax (every other particle: kPlaceAtomic there), and the
+    // cluster stores min(stayers, kPlaceMax) of each of its cells here.  The
+    // sort puts the placed ones at cell start + r and hands out the rest of
+    // the bucket with the atomic count-down; its scan clears this array.
+    uint8_t *stay;     // ncells rounded up to a multiple of 4, zero between calls
 };
+constexpr uint32_t kPlaceMax = 31, kPlaceAtomic = 31;
 int64_t sort_bsum_size(int64_t ncells);
 // Counting sort of particles by destination cell, ties by (direction,
 // slot) when in_cell_order (otherwise the order inside a cell is that of
@@ -141,10 +151,12 @@ int64_t sort_bsum_size(int64_t ncells);
 // then still gives every cluster's range, the cells' sub-ranges refer to
 // cell-major ranks.  The canonical order breaks ties by the source slot, or
 // by tie[slot] when given (the slot's cell-major position in the previous
-// sorted order, for sources stored interleaved).
+// sorted order, for sources stored interleaved).  placed (with counted and
+// not in_cell_order): the keys and sb.stay carry the particle kernel's
+// placed stayers (SortBufs::stay).
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st,
                  bool counted = false, bool in_cell_order = true, bool interleaved = false,
-                 const int32_t *tie = nullptr);
+                 const int32_t *tie = nullptr, bool placed = false);
 // Re-stores the current sorted order in another cluster layout (perm via
 // tmp); keys are those the order was sorted from.
 void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, bool from_interleaved,
@@ -166,6 +178,7 @@ struct MixSpecies {
     const int32_t *perm, *cell_off;  // sorted view of the input
     int32_t *counts;                // destination histogram
     int32_t *rank;                  // or null: cell-major input position of each output slot
+    uint8_t *stay;                  // or null: placed stayers per cell (SortBufs::stay)
     int lay;                        // the input's clusters are stored lane-interleaved
 };
 // Particle kernel of the mixed form over ns <= kMaxMerged species that share

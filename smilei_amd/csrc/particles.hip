@@ -135,6 +135,7 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
     const gptr<uint32_t> keys = glb(Sp.keys);
     int32_t *__restrict__ counts = Sp.counts;
     const gptr<int32_t> rank = glb(Sp.rank);
+    const bool stay = Sp.stay != nullptr;
     const int lay = Sp.lay;
     const gptr<double> scratch = glb(Sp.scratch);
     const int64_t cap = Sp.cap;
@@ -175,8 +176,8 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
         if (!inside) {
             // not in this cluster (corrupt state): leave it untouched, flag it
             atomicOr(err, 1);
-            keys[jslot] = (13u << 27) | (uint32_t)(cl.gbase);
-            atomicAdd(&sHist[sid][0], 1);
+            keys[jslot] = ((stay ? kPlaceAtomic : 13u) << 27) | (uint32_t)(cl.gbase);
+            atomicAdd(&counts[cl.gbase], 1);
             B.x[jslot] = x[0];
             B.y[jslot] = x[1];
             if constexpr (ND == 3) B.z[jslot] = x[2];
@@ -230,13 +231,24 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
         }
         const int dp = (dpc[0] * g.npatch[1] + dpc[1]) * g.npatch[2] + dpc[2];
         const uint32_t gkey = (uint32_t)((int64_t)dp * g.cpp + cell_key<ND>(g, dpc, xw));
-        const uint32_t dir = (uint32_t)((m[0] + 1) * 9 + (m[1] + 1) * 3 + (m[2] + 1));
-        keys[jslot] = (dir << 27) | gkey;
         // histogram of destination cells for the sort: own-cluster cells in
-        // LDS, particles leaving the cluster straight to the global counts
+        // LDS, particles leaving the cluster straight to the global counts.
+        // Top key bits: the direction of motion, or with `stay` the rank
+        // among this cluster's stayers of the cell (placed by the sort).
+        uint32_t tag = (uint32_t)((m[0] + 1) * 9 + (m[1] + 1) * 3 + (m[2] + 1));
         const int64_t rel_cell = (int64_t)gkey - cl.gbase;
-        if (rel_cell >= 0 && rel_cell < g.cells_per_cl) atomicAdd(&sHist[sid][rel_cell], 1);
-        else atomicAdd(&counts[gkey], 1);
+        if (rel_cell >= 0 && rel_cell < g.cells_per_cl) {
+            if (stay) {
+                const int r = atomicAdd(&sHist[sid][rel_cell], 1);
+                tag = r < (int)kPlaceMax ? (uint32_t)r : kPlaceAtomic;
+            } else {
+                atomicAdd(&sHist[sid][rel_cell], 1);
+            }
+        } else {
+            atomicAdd(&counts[gkey], 1);
+            if (stay) tag = kPlaceAtomic;
+        }
+        keys[jslot] = (tag << 27) | gkey;
         B.x[jslot] = xw[0];
         B.y[jslot] = xw[1];
         if constexpr (ND == 3) B.z[jslot] = xw[2];
@@ -308,7 +320,11 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
     __syncthreads();
     for (int t = threadIdx.x; t < ns * 64; t += blockDim.x) {
         const int c = t % 64 < g.cells_per_cl ? sHist[t / 64][t % 64] : 0;
-        if (c) atomicAdd(&specs[t / 64].counts[cl.gbase + t % 64], c);
+        if (c) {
+            const FpSpecies &S = specs[t / 64];
+            atomicAdd(&S.counts[cl.gbase + t % 64], c);
+            if (S.stay) S.stay[cl.gbase + t % 64] = (uint8_t)(c < (int)kPlaceMax ? c : (int)kPlaceMax);
+        }
     }
     if constexpr (QUEUE) {
         const int nq = sQn < QCAP ? sQn : QCAP;

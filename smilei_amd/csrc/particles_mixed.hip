@@ -365,6 +365,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     const gptr<const int32_t> perm = glb(S.perm);
     int32_t *__restrict__ counts = S.counts;
     const gptr<int32_t> rank = glb(S.rank);
+    const bool stay = S.stay != nullptr;
     const int lay = S.lay;
     const int s = S.cell_off[cl.gbase], e = S.cell_off[cl.gbase + 64];
     const int N = e - s;
@@ -380,11 +381,11 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const float w = A.w[i];
         if (lc < 0 || lc >= 64) {  // not in this cluster: corrupt state
             atomicOr(err, 1);
-            keysB[jslot] = (13u << 27) | (uint32_t)cl.gbase;
+            keysB[jslot] = ((stay ? kPlaceAtomic : 13u) << 27) | (uint32_t)cl.gbase;
             B.ox[jslot] = o[0]; B.oy[jslot] = o[1]; B.oz[jslot] = o[2];
             B.ux[jslot] = u[0]; B.uy[jslot] = u[1]; B.uz[jslot] = u[2];
             B.w[jslot] = w;
-            atomicAdd(&sHist[sid][0], 1);
+            atomicAdd(&counts[cl.gbase], 1);
             continue;
         }
         const int l[3] = {lc >> 4, (lc >> 2) & 3, lc & 3};  // node in the cluster
@@ -515,8 +516,22 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const int dcl = ((lp[0] >> 2) * g.ncl[1] + (lp[1] >> 2)) * g.ncl[2] + (lp[2] >> 2);
         const uint32_t gkey =
             (uint32_t)((int64_t)dp * g.cpp + dcl * 64 + 16 * (lp[0] & 3) + 4 * (lp[1] & 3) + (lp[2] & 3));
-        const uint32_t dir = (uint32_t)((mdir[0] + 1) * 9 + (mdir[1] + 1) * 3 + (mdir[2] + 1));
-        keysB[jslot] = (dir << 27) | gkey;
+        // top key bits: the direction of motion, or with `stay` the rank
+        // among this cluster's stayers of the cell (placed by the sort)
+        uint32_t tag = (uint32_t)((mdir[0] + 1) * 9 + (mdir[1] + 1) * 3 + (mdir[2] + 1));
+        const int64_t rc = (int64_t)gkey - cl.gbase;
+        if (rc >= 0 && rc < 64) {
+            if (stay) {
+                const int r = atomicAdd(&sHist[sid][rc], 1);
+                tag = r < (int)kPlaceMax ? (uint32_t)r : kPlaceAtomic;
+            } else {
+                atomicAdd(&sHist[sid][rc], 1);
+            }
+        } else {
+            atomicAdd(&counts[gkey], 1);
+            if (stay) tag = kPlaceAtomic;
+        }
+        keysB[jslot] = (tag << 27) | gkey;
         B.ox[jslot] = on[0];
         B.oy[jslot] = on[1];
         B.oz[jslot] = on[2];
@@ -524,16 +539,17 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         B.uy[jslot] = un[1];
         B.uz[jslot] = un[2];
         B.w[jslot] = w;
-        const int64_t rc = (int64_t)gkey - cl.gbase;
-        if (rc >= 0 && rc < 64) atomicAdd(&sHist[sid][rc], 1);
-        else atomicAdd(&counts[gkey], 1);
     }
     }  // species
 
     __syncthreads();
     for (int t = threadIdx.x; t < ns * 64; t += blockDim.x) {
         const int c = sHist[t / 64][t % 64];
-        if (c) atomicAdd(&specs[t / 64].counts[cl.gbase + t % 64], c);
+        if (c) {
+            const MixSpecies &S = specs[t / 64];
+            atomicAdd(&S.counts[cl.gbase + t % 64], c);
+            if (S.stay) S.stay[cl.gbase + t % 64] = (uint8_t)(c < (int)kPlaceMax ? c : (int)kPlaceMax);
+        }
     }
     // deferred deposits of the node changers
     const int nq = sQn < QCAP ? sQn : QCAP;

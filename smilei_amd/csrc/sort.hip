@@ -15,6 +15,7 @@
 // The particle data itself is not moved here: the next step's particle
 // kernel reads its input through `perm`.
 #include "kernels.h"
+#include "pic_device.h"
 
 namespace smi {
 
@@ -100,10 +101,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_bsum(int32_t *__restrict_
 // out[i] = bsum[block] + exclusive prefix inside the block; out[n] = total.
 // Each thread owns 16 consecutive entries, moved as four 16-byte vectors
 // (the entries of a full tile are 64-byte aligned); the ragged last tile
-// takes the scalar path.
+// takes the scalar path.  stay (placed stayers, or null): in[i] -= stay[i]
+// after the totals are read, leaving the count of the bucket's atomic
+// places, and stay[i] = 0.
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_apply(const int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bsum,
-             int32_t *__restrict__ out)
+k_scan_apply(int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bsum, int32_t *__restrict__ out,
+             uint8_t *__restrict__ stay)
 {
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     const bool full = (int64_t)(blockIdx.x + 1) * kScanTile <= n;
@@ -146,20 +149,53 @@ k_scan_apply(const int32_t *__restrict__ in, int64_t n, const int32_t *__restric
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
         if (base + k == n - 1) out[n] = o[k] + v[k];
+    if (!stay) return;
+    if (full) {
+        uint4 *sp = reinterpret_cast<uint4 *>(stay + base);
+        const uint4 t = *sp;
+        const uint32_t wd[4] = {t.x, t.y, t.z, t.w};
+        if (wd[0] | wd[1] | wd[2] | wd[3]) {
+            int4 *dst = reinterpret_cast<int4 *>(in + base);
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q)
+                dst[q] = make_int4(v[4 * q] - (int)(wd[q] & 255u), v[4 * q + 1] - (int)((wd[q] >> 8) & 255u),
+                                   v[4 * q + 2] - (int)((wd[q] >> 16) & 255u), v[4 * q + 3] - (int)(wd[q] >> 24));
+            *sp = make_uint4(0u, 0u, 0u, 0u);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n && stay[base + k]) {
+                in[base + k] = v[k] - stay[base + k];
+                stay[base + k] = 0;
+            }
+    }
 }
 
 // Scatter of slot indices into their cell buckets.  Consecutive slots mostly
 // keep their cell, so lanes of a wavefront with equal keys form runs: the
 // run head takes `run` places with one atomic and the lanes fill them in
-// slot order.  The count-down returns the histogram to zero.  cpc > 0 (the
-// cells per cluster): the cell-major place is stored at its lane-
-// interleaved slot of the cluster (cluster_slot).
+// slot order.  The count-down returns the histogram to zero.  With the
+// count c before a run's atomic, the run takes places [end - c, end - c +
+// run) of its bucket [start, end): successive runs fill the top counts[g]
+// places, the whole bucket without placed stayers.  placed: a key whose top
+// bits hold a rank r < kPlaceAtomic goes to start + r with no atomic (its
+// cluster's stayers, SortBufs::stay).  cpc > 0 (the cells per cluster): the
+// cell-major place is stored at its lane-interleaved slot of the cluster
+// (cluster_slot).  The slots are dealt to the workgroups in the XCD-aware
+// order (xcd_cluster): consecutive slots, and so the writes into one
+// cluster's range of perm, meet in one L2, where the partial lines merge
+// before they are written back.
 __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
-                          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc)
+                          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc, int placed)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)dev::xcd_cluster(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const uint32_t g = i < n ? keys[i] & kKeyMask : 0xFFFFFFFFu;
+    const uint32_t kk = i < n ? keys[i] : 0u;
+    const uint32_t tag = kk >> 27;
+    const bool direct = placed && tag < kPlaceAtomic;
+    // lanes of the count-down; the others carry the sentinel
+    const uint32_t g = i < n && !direct ? kk & kKeyMask : 0xFFFFFFFFu;
     const uint32_t gp = __shfl_up(g, 1, 64);
     const bool head = lane == 0 || gp != g;
     const uint64_t heads = __ballot(head);
@@ -169,13 +205,11 @@ __global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const in
     int c = 0;
     if (head && g != 0xFFFFFFFFu) c = atomicSub(&counts[g], next - lane);
     const int ch = __shfl(c, my_head, 64);
-    const int run_next = __shfl(next, my_head, 64);
-    if (g == 0xFFFFFFFFu) return;
-    // the run occupies places [c - run, c) counted from the bucket start
-    const int run = run_next - my_head;
-    const int pos = cell_off[g] + ch - run + (lane - my_head);
+    if (i >= n) return;
+    const uint32_t gc = kk & kKeyMask;
+    const int pos = direct ? cell_off[gc] + (int)tag : cell_off[gc + 1] - ch + (lane - my_head);
     if (cpc) {
-        const uint32_t cb = g & ~(uint32_t)(cpc - 1);
+        const uint32_t cb = gc & ~(uint32_t)(cpc - 1);
         const int s = cell_off[cb];
         perm[s + cluster_slot(pos - s, cell_off[cb + cpc] - s)] = (int32_t)i;
     } else {
@@ -422,8 +456,9 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 int64_t sort_bsum_size(int64_t ncells) { return (ncells + kScanTile - 1) / kScanTile + 1; }
 
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st, bool counted,
-                 bool in_cell_order, bool interleaved, const int32_t *tie)
+                 bool in_cell_order, bool interleaved, const int32_t *tie, bool placed)
 {
+    placed = placed && counted && !in_cell_order;
     const int64_t nc = g.ncells;
     if (n > 0 && !counted) {
         hipLaunchKernelGGL(k_hist, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, sb.counts);
@@ -435,8 +470,8 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
     SMI_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kScanThreads), 0, st, sb.bsum, nb);
     SMI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, st, (const int32_t *)sb.counts, nc,
-                       (const int32_t *)sb.bsum, sb.cell_off);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, st, sb.counts, nc,
+                       (const int32_t *)sb.bsum, sb.cell_off, placed ? sb.stay : nullptr);
     SMI_LAUNCH_CHECK();
     if (n > 0) {
         // cell-major places go straight to perm unless the canonical order
@@ -445,7 +480,7 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
         const int cpc = interleaved ? g.cells_per_cl : 0;
         int32_t *first = in_cell_order && interleaved ? sb.tmp : sb.perm;
         hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
-                           sb.counts, first, in_cell_order ? 0 : cpc);
+                           sb.counts, first, in_cell_order ? 0 : cpc, placed ? 1 : 0);
         SMI_LAUNCH_CHECK();
         if (in_cell_order) {
             if (sb.big) launch_fill_i32(sb.nbig, 2, 0, st);

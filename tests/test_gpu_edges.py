@@ -145,3 +145,62 @@ def test_positions_on_cell_faces_and_box_end(precision):
     sim.step(1)
     after = sim.get_particles(s)
     np.testing.assert_array_equal(after["patch"], got["patch"])
+
+
+def cell_multisets_agree(a, b, s, tol):
+    """Same particles per (patch, cell) in both simulations, positions and
+    momenta to tol relative to their largest magnitude."""
+    pa, pb = a.get_particles(s), b.get_particles(s)
+    np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+    ia = np.lexsort((pa["z"], pa["y"], pa["x"], pa["key"], pa["patch"]))
+    ib = np.lexsort((pb["z"], pb["y"], pb["x"], pb["key"], pb["patch"]))
+    for k in ("patch", "key"):
+        np.testing.assert_array_equal(pa[k][ia], pb[k][ib], err_msg=k)
+    for k in ("x", "y", "z", "px", "py", "pz", "w"):
+        scale = max(np.abs(pb[k]).max(), 1e-300)
+        assert np.abs(pa[k][ia] - pb[k][ib]).max() <= tol * scale, k
+
+
+@pytest.mark.parametrize("interleave", [1, 0])
+@pytest.mark.parametrize("precision", [sa.PLAIN_FP64, sa.MIXED])
+def test_placed_stayers_keep_the_cell_contents(precision, interleave):
+    """In-cell order left to the sort (SMI_TUNE_CELL_ORDER 0): the particle
+    kernel places the first 31 of its cluster's stayers of each cell (their
+    rank in the key's top bits), the others and the particles arriving from
+    other clusters take the atomic count-down.  With 3000 slow particles in
+    one cell (thousands of stayers, most of them past the placed ranks), a
+    switch to the canonical order and back, in both cluster layouts, every
+    cell holds the same particles as in the canonical run."""
+    p = dense_plasma(n_dense=3000, sigma=0.05)
+    a, b = new(precision), new(precision)
+    for sim in (a, b):
+        sim.set_tuning(sa.TUNE_INTERLEAVE, interleave)
+        load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
+    b.set_tuning(sa.TUNE_CELL_ORDER, 0)
+    a.step(3)
+    b.step(3)
+    b.set_tuning(sa.TUNE_CELL_ORDER, 1)
+    a.step(1)
+    b.step(1)
+    b.set_tuning(sa.TUNE_CELL_ORDER, 0)
+    a.step(3)
+    b.step(3)
+    tol = 1e-12 if precision == sa.PLAIN_FP64 else 1e-6
+    cell_multisets_agree(b, a, 0, tol)
+    for f in ("Ex", "Jy", "Bz"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
+
+
+def test_placed_stayers_2d():
+    """The same in 2D (16-cell clusters), FP64 form."""
+    shape, patch, dx = (64, 32), (16, 16), (0.2, 0.2)
+    sims = []
+    for order in (1, 0):
+        sim = sa.Simulation("2Dcartesian", 2, shape, patch, dx, cfl_dt(dx))
+        sim.set_tuning(sa.TUNE_CELL_ORDER, order)
+        p = thermal(shape, dx, 40, 0.05, 11)
+        load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
+        sim.step(5)
+        sims.append(sim)
+    cell_multisets_agree(sims[1], sims[0], 0, 1e-12)
