@@ -185,35 +185,53 @@ k_scan_apply(int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bs
 // (cluster_slot).  The slots are dealt to the workgroups in the XCD-aware
 // order (xcd_cluster): consecutive slots, and so the writes into one
 // cluster's range of perm, meet in one L2, where the partial lines merge
-// before they are written back.
-__global__ void k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
-                          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc, int placed)
+// before they are written back.  Each wavefront takes kScatItems groups of
+// 64 consecutive slots and issues the loads, atomics and stores of the
+// groups together: the kernel is bound by the latency of its dependent
+// accesses (key, count-down, bucket offsets, store), not by bytes.
+constexpr int kScatThreads = 256, kScatItems = 4, kScatTile = kScatThreads * kScatItems;
+
+__global__ void __launch_bounds__(kScatThreads)
+k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
+          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc, int placed)
 {
-    const int64_t i = (int64_t)dev::xcd_cluster(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const uint32_t kk = i < n ? keys[i] : 0u;
-    const uint32_t tag = kk >> 27;
-    const bool direct = placed && tag < kPlaceAtomic;
-    // lanes of the count-down; the others carry the sentinel
-    const uint32_t g = i < n && !direct ? kk & kKeyMask : 0xFFFFFFFFu;
-    const uint32_t gp = __shfl_up(g, 1, 64);
-    const bool head = lane == 0 || gp != g;
-    const uint64_t heads = __ballot(head);
-    const uint64_t above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
-    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
-    const int my_head = 63 - __clzll((unsigned long long)(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1))));
-    int c = 0;
-    if (head && g != 0xFFFFFFFFu) c = atomicSub(&counts[g], next - lane);
-    const int ch = __shfl(c, my_head, 64);
-    if (i >= n) return;
-    const uint32_t gc = kk & kKeyMask;
-    const int pos = direct ? cell_off[gc] + (int)tag : cell_off[gc + 1] - ch + (lane - my_head);
-    if (cpc) {
-        const uint32_t cb = gc & ~(uint32_t)(cpc - 1);
-        const int s = cell_off[cb];
-        perm[s + cluster_slot(pos - s, cell_off[cb + cpc] - s)] = (int32_t)i;
-    } else {
-        perm[pos] = (int32_t)i;
+    const int64_t i0 = (int64_t)dev::xcd_cluster(blockIdx.x, gridDim.x) * kScatTile +
+                       (int64_t)(threadIdx.x >> 6) * (64 * kScatItems) + lane;
+    uint32_t kk[kScatItems];
+#pragma unroll
+    for (int j = 0; j < kScatItems; ++j) kk[j] = i0 + 64 * j < n ? keys[i0 + 64 * j] : 0u;
+    int c[kScatItems], my_head[kScatItems];
+#pragma unroll
+    for (int j = 0; j < kScatItems; ++j) {
+        const bool direct = placed && (kk[j] >> 27) < kPlaceAtomic;
+        // lanes of the count-down; the others carry the sentinel
+        const uint32_t g = i0 + 64 * j < n && !direct ? kk[j] & kKeyMask : 0xFFFFFFFFu;
+        const uint32_t gp = __shfl_up(g, 1, 64);
+        const bool head = lane == 0 || gp != g;
+        const uint64_t heads = __ballot(head);
+        const uint64_t above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+        const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
+        my_head[j] = 63 - __clzll((unsigned long long)(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1))));
+        c[j] = 0;
+        if (head && g != 0xFFFFFFFFu) c[j] = atomicSub(&counts[g], next - lane);
+    }
+#pragma unroll
+    for (int j = 0; j < kScatItems; ++j) {
+        const int ch = __shfl(c[j], my_head[j], 64);
+        const int64_t i = i0 + 64 * j;
+        if (i >= n) continue;
+        const uint32_t tag = kk[j] >> 27;
+        const bool direct = placed && tag < kPlaceAtomic;
+        const uint32_t gc = kk[j] & kKeyMask;
+        const int pos = direct ? cell_off[gc] + (int)tag : cell_off[gc + 1] - ch + (lane - my_head[j]);
+        if (cpc) {
+            const uint32_t cb = gc & ~(uint32_t)(cpc - 1);
+            const int s = cell_off[cb];
+            perm[s + cluster_slot(pos - s, cell_off[cb + cpc] - s)] = (int32_t)i;
+        } else {
+            perm[pos] = (int32_t)i;
+        }
     }
 }
 
@@ -479,7 +497,7 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
         // then they are ordered in tmp and laid out into perm
         const int cpc = interleaved ? g.cells_per_cl : 0;
         int32_t *first = in_cell_order && interleaved ? sb.tmp : sb.perm;
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(n, kScatTile)), dim3(kScatThreads), 0, st, keys, n, (const int32_t *)sb.cell_off,
                            sb.counts, first, in_cell_order ? 0 : cpc, placed ? 1 : 0);
         SMI_LAUNCH_CHECK();
         if (in_cell_order) {
