@@ -239,13 +239,15 @@ k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restric
 // One thread per cell.  Buckets of up to kFixRegs particles are ranked in
 // registers (all loads issued together, rank = number of smaller
 // (direction, slot) pairs; the pairs are distinct because slots are);
-// larger buckets go to the work list `big` (count in nbig[0]) for
+// buckets of up to kFixMid are left to k_bucket_fix_mid, larger ones go to
+// the work list `big` (count in nbig[0], appended by k_bucket_fix_mid) for
 // k_bucket_fix_big, or, without a list, to an insertion sort in memory.
 // The cell-major input is src; cpc == 0: sorted in place (dst == src);
 // cpc > 0: every particle is written to its lane-interleaved slot of the
 // cluster in dst.  The tie-break value of source slot v is tie[v] when tie
 // is given, else v.
 constexpr int kFixRegs = 16;
+constexpr int kFixMid = 64;
 
 __device__ inline uint64_t fix_key(const uint32_t *keys, const int32_t *tie, int32_t v)
 {
@@ -299,10 +301,7 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
         }
         return;
     }
-    if (big) {
-        big[atomicAdd(&nbig[0], 1)] = (int32_t)g;
-        return;
-    }
+    if (big) return;  // k_bucket_fix_mid, k_bucket_fix_big
     int32_t *perm = src;
     for (int i = a + 1; i < b; ++i) {
         const int32_t v = perm[i];
@@ -321,7 +320,58 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
         for (int i = a; i < b; ++i) put(i, perm[i]);
 }
 
-// The cells of the work list (more than kFixRegs particles; their number in
+// Wave-aggregated append of the lanes with `want` to a counter: returns
+// each such lane's index (one atomic per wavefront).
+__device__ inline int wave_append(bool want, int32_t *counter)
+{
+    const uint64_t m = __ballot(want);
+    if (!m) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(m & ((1ull << lane) - 1));
+}
+
+// Each wavefront scans 64 consecutive cells and runs those with kFixRegs <
+// n <= kFixMid particles one after the other, with the whole wavefront:
+// lane l holds place l, its rank = the number of smaller keys over the
+// wavefront (shuffles).  The stores follow the loads of every lane, so the
+// in-place case is safe.  Cells with more particles go to the work list
+// `big` for k_bucket_fix_big.
+__global__ void __launch_bounds__(256)
+k_bucket_fix_mid(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
+                 int32_t *src, int32_t *dst, int cpc, const int32_t *__restrict__ tie, int32_t *__restrict__ big,
+                 int32_t *__restrict__ nbig)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    const int64_t gl = g0 + lane;
+    const int nl = gl < ncells ? cell_off[gl + 1] - cell_off[gl] : 0;
+    const int ib = wave_append(nl > kFixMid, &nbig[0]);
+    if (nl > kFixMid) big[ib] = (int32_t)gl;
+    uint64_t todo = __ballot(nl > kFixRegs && nl <= kFixMid);
+    while (todo) {
+        const int bit = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1;
+        const int64_t g = g0 + bit;
+        const int a = cell_off[g], n = cell_off[g + 1] - a;
+        int cs = 0, cn = 0;
+        if (cpc) {
+            const int64_t cb = g & ~(int64_t)(cpc - 1);
+            cs = cell_off[cb];
+            cn = cell_off[cb + cpc] - cs;
+        }
+        const int32_t v = lane < n ? src[a + lane] : 0;
+        const uint64_t k = lane < n ? fix_key(keys, tie, v) : ~0ull;
+        int r = 0;
+        for (int j = 0; j < n; ++j) r += __shfl(k, j, 64) < k ? 1 : 0;
+        if (lane < n) dst[cpc ? cs + cluster_slot(a + r - cs, cn) : a + r] = v;
+    }
+}
+
+// The cells of the work list (more than kFixMid particles; their number in
 // nbig[0], nbig[1] = 0 on entry), one workgroup per cell pulled from the
 // list until it is exhausted.  Up to kBigLds particles: bitonic sort of the
 // (key, slot) pairs in LDS.  More: each particle's rank = the number of
@@ -506,6 +556,9 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
                                (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie, sb.big, sb.nbig);
             SMI_LAUNCH_CHECK();
             if (sb.big) {
+                hipLaunchKernelGGL(k_bucket_fix_mid, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
+                                   (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie, sb.big, sb.nbig);
+                SMI_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_bucket_fix_big, dim3(kBigBlocks), dim3(kBigThreads), 0, st, keys,
                                    (const int32_t *)sb.cell_off, first, sb.perm, sb.tmp, cpc, tie,
                                    (const int32_t *)sb.big, sb.nbig);
