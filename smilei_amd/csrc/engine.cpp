@@ -468,6 +468,7 @@ int smi_destroy(smi_context *c)
         (void)hipFree(s.sb.nbig);
         (void)hipFree(s.sb.bsum);
         (void)hipFree(s.sb.stay);
+        (void)hipFree(s.sb.clus_old);
         (void)hipFree(s.scratch);
     }
     (void)hipFree(c->pj);
@@ -511,6 +512,7 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     s.sb.nbsum = sort_bsum_size(c->g.ncells);
     s.sb.bsum = dalloc<int32_t>(s.sb.nbsum);
     s.sb.stay = (uint8_t *)dalloc<int32_t>((c->g.ncells + 3) / 4);
+    s.sb.clus_old = dalloc<int32_t>(c->g.ncells / c->g.cells_per_cl);
     launch_fill_i32((int32_t *)s.sb.stay, (c->g.ncells + 3) / 4, 0, c->st);
     if (c->p.deposit == SMI_DEPOSIT_DETERMINISTIC) s.scratch = dalloc<double>(7 * capacity);
     launch_fill_i32(s.sb.counts, c->g.ncells, 0, c->st);

@@ -138,6 +138,7 @@ struct SortBufs {
     // sort puts the placed ones at cell start + r and hands out the rest of
     // the bucket with the atomic count-down; its scan clears this array.
     uint8_t *stay;     // ncells rounded up to a multiple of 4, zero between calls
+    int32_t *clus_old; // clusters, or null: each cluster's first slot in the order before the sort
 };
 constexpr uint32_t kPlaceMax = 31, kPlaceAtomic = 31;
 int64_t sort_bsum_size(int64_t ncells);

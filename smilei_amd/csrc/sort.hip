@@ -103,13 +103,20 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_bsum(int32_t *__restrict_
 // (the entries of a full tile are 64-byte aligned); the ragged last tile
 // takes the scalar path.  stay (placed stayers, or null): in[i] -= stay[i]
 // after the totals are read, leaving the count of the bucket's atomic
-// places, and stay[i] = 0.
+// places, and stay[i] = 0.  clus_old (or null): the previous out[] of every
+// cpcell-th entry (each cluster's first slot before this sort) is saved
+// there before it is overwritten.
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_apply(int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bsum, int32_t *__restrict__ out,
-             uint8_t *__restrict__ stay)
+             uint8_t *__restrict__ stay, int32_t *__restrict__ clus_old, int cpcell)
 {
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     const bool full = (int64_t)(blockIdx.x + 1) * kScanTile <= n;
+    if (clus_old) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n && (base + k) % cpcell == 0) clus_old[(base + k) / cpcell] = out[base + k];
+    }
     int v[kScanItems];
     int s = 0;
     if (full) {
@@ -249,14 +256,52 @@ k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restric
 constexpr int kFixRegs = 16;
 constexpr int kFixMid = 64;
 
-__device__ inline uint64_t fix_key(const uint32_t *keys, const int32_t *tie, int32_t v)
+// Where the tie-breaks come from: tie[v] (the cell-major position of slot
+// v in the previous sorted order, recorded by the particle kernel), or v
+// itself when tie is null.  With clus_old (each cluster's first slot in
+// the previous order, stored lane-interleaved), the tie of a slot inside
+// the previous range of the cell's own cluster -- a particle that stayed
+// in its cluster, nearly all of them -- is computed (cluster_rank) instead
+// of gathered.  Such a particle did not leave its patch, so its direction of
+// motion is 13 (no patch face crossed) and its key need not be read either,
+// unless some axis is one cluster wide (a periodic wrap lands in the same
+// cluster): stay13 = 0 then.
+struct TieSrc {
+    const int32_t *tie;
+    const int32_t *clus_old;
+    int cpcell, ncl;
+    int64_t n;
+    int stay13;
+    // previous range [cs, cs + cn) of the cluster of cell g (cn = 0: none)
+    __device__ void range(int64_t g, int &cs, int &cn) const
+    {
+        cs = 0;
+        cn = 0;
+        if (!tie || !clus_old) return;
+        const int b = (int)(g / cpcell);
+        cs = clus_old[b];
+        cn = (b + 1 < ncl ? clus_old[b + 1] : (int)n) - cs;
+    }
+};
+
+__device__ inline uint64_t fix_key(const uint32_t *keys, const TieSrc &T, int32_t v, int cs, int cn)
 {
-    return ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)(tie ? tie[v] : v);
+    uint32_t t = (uint32_t)v;
+    if (T.tie) {
+        const int r = v - cs;
+        if (r >= 0 && r < cn) {
+            t = (uint32_t)(cs + cluster_rank(r, cn));
+            if (T.stay13) return (13ull << 32) | t;
+        } else {
+            t = (uint32_t)T.tie[v];
+        }
+    }
+    return ((uint64_t)(keys[v] >> 27) << 32) | t;
 }
 
 __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
-                             int32_t *src, int32_t *dst, int cpc, const int32_t *__restrict__ tie,
-                             int32_t *__restrict__ big, int32_t *__restrict__ nbig)
+                             int32_t *src, int32_t *dst, int cpc, TieSrc T, int32_t *__restrict__ big,
+                             int32_t *__restrict__ nbig)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncells) return;
@@ -274,14 +319,15 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
         if (cpc && n == 1) put(a, src[a]);
         return;
     }
+    int ocs, ocn;
+    T.range(g, ocs, ocn);
     if (n <= kFixRegs) {
         int32_t v[kFixRegs];
         uint64_t k[kFixRegs];
 #pragma unroll
         for (int i = 0; i < kFixRegs; ++i) v[i] = i < n ? src[a + i] : 0;
 #pragma unroll
-        for (int i = 0; i < kFixRegs; ++i)
-            k[i] = i < n ? ((uint64_t)(keys[v[i]] >> 27) << 32) | (uint32_t)(tie ? tie[v[i]] : v[i]) : ~0ull;
+        for (int i = 0; i < kFixRegs; ++i) k[i] = i < n ? fix_key(keys, T, v[i], ocs, ocn) : ~0ull;
         bool sorted = true;
 #pragma unroll
         for (int i = 1; i < kFixRegs; ++i) sorted &= k[i - 1] < k[i] || i >= n;
@@ -305,11 +351,11 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
     int32_t *perm = src;
     for (int i = a + 1; i < b; ++i) {
         const int32_t v = perm[i];
-        const uint64_t kv = ((uint64_t)(keys[v] >> 27) << 32) | (uint32_t)(tie ? tie[v] : v);
+        const uint64_t kv = fix_key(keys, T, v, ocs, ocn);
         int j = i - 1;
         while (j >= a) {
             const int32_t u = perm[j];
-            const uint64_t ku = ((uint64_t)(keys[u] >> 27) << 32) | (uint32_t)(tie ? tie[u] : u);
+            const uint64_t ku = fix_key(keys, T, u, ocs, ocn);
             if (ku <= kv) break;
             perm[j + 1] = u;
             --j;
@@ -342,7 +388,7 @@ __device__ inline int wave_append(bool want, int32_t *counter)
 // `big` for k_bucket_fix_big.
 __global__ void __launch_bounds__(256)
 k_bucket_fix_mid(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
-                 int32_t *src, int32_t *dst, int cpc, const int32_t *__restrict__ tie, int32_t *__restrict__ big,
+                 int32_t *src, int32_t *dst, int cpc, TieSrc T, int32_t *__restrict__ big,
                  int32_t *__restrict__ nbig)
 {
     const int lane = threadIdx.x & 63;
@@ -363,8 +409,10 @@ k_bucket_fix_mid(const uint32_t *__restrict__ keys, int64_t ncells, const int32_
             cs = cell_off[cb];
             cn = cell_off[cb + cpc] - cs;
         }
+        int ocs, ocn;
+        T.range(g, ocs, ocn);
         const int32_t v = lane < n ? src[a + lane] : 0;
-        const uint64_t k = lane < n ? fix_key(keys, tie, v) : ~0ull;
+        const uint64_t k = lane < n ? fix_key(keys, T, v, ocs, ocn) : ~0ull;
         int r = 0;
         for (int j = 0; j < n; ++j) r += __shfl(k, j, 64) < k ? 1 : 0;
         if (lane < n) dst[cpc ? cs + cluster_slot(a + r - cs, cn) : a + r] = v;
@@ -386,8 +434,7 @@ constexpr int kBigBlocks = 1024;
 
 __global__ void __launch_bounds__(kBigThreads)
 k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ cell_off, int32_t *src, int32_t *dst,
-                 int32_t *scr, int cpc, const int32_t *__restrict__ tie, const int32_t *__restrict__ big,
-                 int32_t *__restrict__ nbig)
+                 int32_t *scr, int cpc, TieSrc T, const int32_t *__restrict__ big, int32_t *__restrict__ nbig)
 {
     __shared__ uint64_t sK[kBigLds];
     __shared__ int32_t sV[kBigLds];
@@ -409,12 +456,14 @@ k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ 
             cn = cell_off[cb + cpc] - cs;
         }
         auto put = [&](int pos, int32_t v) { dst[cpc ? cs + cluster_slot(pos - cs, cn) : pos] = v; };
+        int ocs, ocn;
+        T.range(g, ocs, ocn);
         if (n <= kBigLds) {
             int P = 1;
             while (P < n) P <<= 1;
             for (int i = tid; i < P; i += kBigThreads) {
                 const int32_t v = i < n ? src[a + i] : 0;
-                sK[i] = i < n ? fix_key(keys, tie, v) : ~0ull;
+                sK[i] = i < n ? fix_key(keys, T, v, ocs, ocn) : ~0ull;
                 sV[i] = v;
             }
             __syncthreads();
@@ -451,13 +500,13 @@ k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ 
             for (int m = 0; m < kBigPer; ++m) {
                 const int i = t0 + tid + m * kBigThreads;
                 mv[m] = i < n ? rd[a + i] : 0;
-                mk[m] = i < n ? fix_key(keys, tie, mv[m]) : ~0ull;
+                mk[m] = i < n ? fix_key(keys, T, mv[m], ocs, ocn) : ~0ull;
                 rk[m] = 0;
             }
             for (int c0 = 0; c0 < n; c0 += kBigLds) {
                 const int cnum = n - c0 < kBigLds ? n - c0 : kBigLds;
                 __syncthreads();
-                for (int i = tid; i < cnum; i += kBigThreads) sK[i] = fix_key(keys, tie, rd[a + c0 + i]);
+                for (int i = tid; i < cnum; i += kBigThreads) sK[i] = fix_key(keys, T, rd[a + c0 + i], ocs, ocn);
                 __syncthreads();
                 for (int j = 0; j < cnum; ++j) {
                     const uint64_t kj = sK[j];
@@ -538,8 +587,16 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
     SMI_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kScanThreads), 0, st, sb.bsum, nb);
     SMI_LAUNCH_CHECK();
+    // the in-cell pass of the step's sort computes the ties of the particles
+    // that stayed in their cluster from the clusters' previous ranges
+    int stay13 = 1;
+    for (int d = 0; d < g.nd; ++d)
+        if (g.npatch[d] * g.n[d] <= 4) stay13 = 0;
+    const TieSrc T{tie, counted && tie && in_cell_order ? sb.clus_old : nullptr, g.cells_per_cl,
+                   (int)(nc / g.cells_per_cl), n, stay13};
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, st, sb.counts, nc,
-                       (const int32_t *)sb.bsum, sb.cell_off, placed ? sb.stay : nullptr);
+                       (const int32_t *)sb.bsum, sb.cell_off, placed ? sb.stay : nullptr,
+                       const_cast<int32_t *>(T.clus_old), g.cells_per_cl);
     SMI_LAUNCH_CHECK();
     if (n > 0) {
         // cell-major places go straight to perm unless the canonical order
@@ -553,14 +610,14 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
         if (in_cell_order) {
             if (sb.big) launch_fill_i32(sb.nbig, 2, 0, st);
             hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie, sb.big, sb.nbig);
+                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, T, sb.big, sb.nbig);
             SMI_LAUNCH_CHECK();
             if (sb.big) {
                 hipLaunchKernelGGL(k_bucket_fix_mid, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                                   (const int32_t *)sb.cell_off, first, sb.perm, cpc, tie, sb.big, sb.nbig);
+                                   (const int32_t *)sb.cell_off, first, sb.perm, cpc, T, sb.big, sb.nbig);
                 SMI_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_bucket_fix_big, dim3(kBigBlocks), dim3(kBigThreads), 0, st, keys,
-                                   (const int32_t *)sb.cell_off, first, sb.perm, sb.tmp, cpc, tie,
+                                   (const int32_t *)sb.cell_off, first, sb.perm, sb.tmp, cpc, T,
                                    (const int32_t *)sb.big, sb.nbig);
                 SMI_LAUNCH_CHECK();
             }

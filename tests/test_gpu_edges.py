@@ -204,3 +204,29 @@ def test_placed_stayers_2d():
         sim.step(5)
         sims.append(sim)
     cell_multisets_agree(sims[1], sims[0], 0, 1e-12)
+
+
+@pytest.mark.parametrize("shape,patch", [((16, 16, 16), (8, 8, 8)), ((16, 16, 8), (8, 8, 8))])
+def test_canonical_order_independent_of_layout(shape, patch):
+    """The canonical in-cell order (direction of motion, then the previous
+    cell-major position) is the same with the clusters stored
+    lane-interleaved -- where the in-cell pass computes the tie-break and
+    the direction of the particles that stayed in their cluster instead of
+    reading them -- as cell-major; also in a box of one 8-cell patch along
+    z (two clusters), where particles wrap around the periodic box from one
+    cluster into the other through their own patch's faces."""
+    dx = (0.2, 0.2, 0.2)
+    sims = []
+    for lay in (1, 0):
+        sim = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx))
+        sim.set_tuning(sa.TUNE_INTERLEAVE, lay)
+        p = thermal(shape, dx, 12, 0.3, 21)
+        load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
+        sim.step(6)
+        sims.append(sim)
+    a, b = sims[0].get_particles(0), sims[1].get_particles(0)
+    for k in ("patch", "key"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    for k in ("x", "y", "z", "px", "py", "pz"):
+        scale = np.abs(b[k]).max()
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * scale, k
