@@ -47,7 +47,7 @@ struct FpSpecies {
     uint32_t *keys;                  // destination keys written for B
     int32_t *counts;                 // destination histogram
     double *scratch;                 // 7*cap doubles (deterministic deposit)
-    int32_t *rank;                   // or null: cell-major input position of each output slot
+    int32_t *rank;                   // or null: cell-major input position of the output slots that left their cluster
     uint8_t *stay;                   // or null: placed stayers per cell (SortBufs::stay)
     int lay;                         // the input's clusters are stored lane-interleaved
     int64_t cap;
@@ -125,7 +125,7 @@ struct SortBufs {
     int32_t *cell_off; // ncells + 1
     int32_t *perm;     // cap
     int32_t *tmp;      // cap (cell-major order before the interleaved layout)
-    int32_t *rank;     // cap (tie-break of the canonical order: input position, cell-major)
+    int32_t *rank;     // cap (tie-break of the canonical order of the particles that left their cluster)
     int32_t *big;      // cap / (kFixRegs + 1) + 1, or null: cells too full for the register ranking
     int32_t *nbig;     // 2 (count, queue head of `big`)
     int32_t *bsum;     // scan scratch
@@ -178,7 +178,7 @@ struct MixSpecies {
     uint32_t *keysB;                // keys written for the output buffer
     const int32_t *perm, *cell_off;  // sorted view of the input
     int32_t *counts;                // destination histogram
-    int32_t *rank;                  // or null: cell-major input position of each output slot
+    int32_t *rank;                  // or null: cell-major input position of the output slots that left their cluster
     uint8_t *stay;                  // or null: placed stayers per cell (SortBufs::stay)
     int lay;                        // the input's clusters are stored lane-interleaved
 };

@@ -146,9 +146,6 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
     for (int rel = threadIdx.x; rel < N; rel += blockDim.x) {
         const int jslot = s + rel;
         const int i = perm[jslot];
-        // cell-major position of this input particle: the tie-break of the
-        // canonical in-cell order at the next sort
-        if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
         double x[3] = {A.x[i], A.y[i], ND == 3 ? A.z[i] : 0.};
         double u[3] = {A.px[i], A.py[i], A.pz[i]};
         const double w = A.w[i];
@@ -247,6 +244,11 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
         } else {
             atomicAdd(&counts[gkey], 1);
             if (stay) tag = kPlaceAtomic;
+            // cell-major position of this input particle, the tie-break of
+            // the canonical in-cell order at the next sort: only read for
+            // the particles that leave their cluster (the sort computes it
+            // for the others)
+            if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
         }
         keys[jslot] = (tag << 27) | gkey;
         B.x[jslot] = xw[0];

@@ -372,9 +372,6 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     for (int rel = threadIdx.x; rel < N; rel += blockDim.x) {
         const int jslot = s + rel;
         const int i = perm[jslot];
-        // cell-major position of this input particle: the tie-break of the
-        // canonical in-cell order at the next sort
-        if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
         const int lc = (int)((int64_t)(keysA[i] & kMask) - cl.gbase);
         float o[3] = {A.ox[i], A.oy[i], A.oz[i]};
         float u[3] = {A.ux[i], A.uy[i], A.uz[i]};
@@ -530,6 +527,11 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         } else {
             atomicAdd(&counts[gkey], 1);
             if (stay) tag = kPlaceAtomic;
+            // cell-major position of this input particle, the tie-break of
+            // the canonical in-cell order at the next sort: only read for
+            // the particles that leave their cluster (the sort computes it
+            // for the others)
+            if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
         }
         keysB[jslot] = (tag << 27) | gkey;
         B.ox[jslot] = on[0];
