@@ -51,7 +51,7 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
-TRAFFIC_PER_PUSH = {"beam": {"mixed": 126.6, "fp64": 181.1}}
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 127.1, "fp64": 186.6}}
 
 
 def cfl_dt(dx, factor=0.95):
