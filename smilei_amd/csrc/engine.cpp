@@ -609,6 +609,7 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
     SMI_CATCH
 }
 
+extern "C++" {  // host helpers of the entry points below, C++ linkage
 namespace {
 
 // Quantiles of |u| = |gamma beta| of the isotropic Maxwell-Juttner
@@ -698,6 +699,7 @@ double cell_volume_of(const Geom &g)
 }
 
 }  // namespace
+}  // extern "C++"
 
 namespace {
 
