@@ -106,10 +106,13 @@ def cylinder_cells(shape, radius):
     return int(np.count_nonzero(j[:, None] ** 2 + k[None, :] ** 2 <= radius * radius)) * shape[0]
 
 
-def run_gpu(args, W, form):
+def make_sim(args, W, form):
+    """The workload's simulation in one form, particles created in HBM from
+    fixed seeds (both forms draw the same particles; the mixed form stores
+    them rounded to its FP32 offsets and momenta)."""
     import smilei_amd as sa
     prec = sa.MIXED if form == "mixed" else sa.PLAIN_FP64
-    sim = sa.Simulation("3Dcartesian", 2, W["shape"], W["patch"], W["dx"], W["dt"],
+    sim = sa.Simulation("3Dcartesian", args.order, W["shape"], W["patch"], W["dx"], W["dt"],
                         precision=prec, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
     sim.set_tuning(sa.TUNE_CELL_ORDER, 1 if args.cell_order == "canonical" else 0)
     ncell = int(np.prod(W["shape"]))
@@ -119,6 +122,11 @@ def run_gpu(args, W, form):
         sid = sim.add_species(name, m, q, cells * ppc)
         sim.init_species(sid, ppc, dens, sig, drift_ux=drift, radius=radius, seed=1 + k)
         per_species.append(sim.num_particles(sid))
+    return sim, per_species
+
+
+def run_gpu(args, W, form):
+    sim, per_species = make_sim(args, W, form)
     npart = sum(per_species)
     e0 = sim.energies()
     sim.step(args.warmup)
@@ -142,6 +150,45 @@ def run_gpu(args, W, form):
     sim.close()
     return dict(npart=npart, per_species=per_species, wall=t1 - t0, wall_graph=tg1 - tg0, kernel_ms=acc,
                 energies=e.tolist(), energies0=e0.tolist(), tuned=tuned)
+
+
+def run_accuracy(args, W):
+    """Both forms of the workload from the same seeds for args.accuracy
+    steps (energies every 100 steps): the mixed form's energies against the
+    FP64 form's, relative to the initial total, and each form's Gauss-law
+    drift max|(div E - rho)(t) - (div E - rho)(0)| / max|rho(0)|."""
+    hist = {}
+    for form in ("mixed", "fp64"):
+        sim, per_species = make_sim(args, W, form)
+        g0 = sim.gauss()
+        rho0 = float(np.abs(sim.rho()).max())
+        e = [sim.energies().tolist()]
+        done = 0
+        while done < args.accuracy:
+            k = min(100, args.accuracy - done)
+            sim.step(k)
+            done += k
+            e.append(sim.energies().tolist())
+            print("accuracy %s: step %d, energies %s" % (form, done, e[-1]), file=sys.stderr, flush=True)
+        drift = float(np.abs(sim.gauss() - g0).max() / rho0)
+        hist[form] = dict(energies=np.array(e), gauss_drift=drift, npart=sum(per_species))
+        sim.close()
+    em, ef = hist["mixed"]["energies"], hist["fp64"]["energies"]
+    tot0 = abs(ef[0].sum())
+    out = {
+        "accuracy_steps": args.accuracy,
+        "workload": W["desc"],
+        "order": args.order,
+        "particles": hist["mixed"]["npart"],
+        "field_energy_error": float(np.abs(em[:, 0] - ef[:, 0]).max() / tot0),
+        "kinetic_energy_error": float(np.abs(em[:, 1:].sum(1) - ef[:, 1:].sum(1)).max() / tot0),
+        "total_energy_error": float(np.abs(em.sum(1) - ef.sum(1)).max() / tot0),
+        "gauss_drift_mixed": hist["mixed"]["gauss_drift"],
+        "gauss_drift_fp64": hist["fp64"]["gauss_drift"],
+        "energy_change_mixed": float(em[-1].sum() / em[0].sum() - 1),
+        "energy_change_fp64": float(ef[-1].sum() / ef[0].sum() - 1),
+    }
+    print(json.dumps(out))
 
 
 def copy_probe(nbytes=4 << 30, reps=10):
@@ -175,7 +222,7 @@ def run_cpu_baseline(args, W):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     O.set_threads(threads)
     shape = (64, 64, 64) if W["name"] == "thermal" else (128, 64, 64)
-    S = O.OracleSim(3, 2, shape, W["patch"], W["dx"], W["dt"])
+    S = O.OracleSim(3, args.order, shape, W["patch"], W["dx"], W["dt"])
     n = 0
     for k, (name, m, q, sig, ppc, dens, drift, radius) in enumerate(W["species"]):
         p = thermal(shape, W["dx"], ppc, sig, 100 + k, density=dens, drift_ux=drift, radius=radius)
@@ -210,6 +257,10 @@ def main():
     ap.add_argument("--forms", default="mixed,fp64", help="comma list of mixed,fp64; the first is the headline")
     ap.add_argument("--copy-only", action="store_true", help="run the device copy probe only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--order", type=int, default=2, choices=[2, 4], help="interpolation order")
+    ap.add_argument("--accuracy", type=int, default=0,
+                    help="instead of timing: run both forms N steps from the same seeds and print the mixed "
+                         "form's energy errors and both forms' Gauss-law drift")
     ap.add_argument("--cell-order", default="any", choices=["any", "canonical"],
                     help="order of particles inside a cell after the sort: 'any' (left to the atomic scatter, "
                          "as the reference's GPU sort) or the canonical order")
@@ -222,6 +273,9 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     W = workload(args.workload, args.ppc)
+    if args.accuracy > 0:
+        run_accuracy(args, W)
+        return
     forms = [f for f in args.forms.split(",") if f]
     copy_gbs = copy_probe()
     res = {}
@@ -257,8 +311,8 @@ def main():
         "dtype": "f32" if head == "mixed" else "f64",
         "data": "synthetic (seeded plasma initialised in HBM)",
         "config": {
-            "workload": "%s (%d particles), order 2, Yee, periodic, atomic tile deposit, in-cell order %s"
-                        % (W["desc"], h["npart"], args.cell_order),
+            "workload": "%s (%d particles), order %d, Yee, periodic, atomic tile deposit, in-cell order %s"
+                        % (W["desc"], h["npart"], args.order, args.cell_order),
             "form": head,
             "particles": h["npart"],
             "forms": {f: {k: v for k, v in r.items() if k != "energies"} for f, r in res.items()},
