@@ -61,9 +61,12 @@ __device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, c
 // (specs[0..ns)) of cluster `cluster` against one field tile (and, with
 // DEPOSIT, one J tile).  Uniform across the workgroup; returns at once for
 // an empty cluster.
+// Inlined into its kernels: left to itself the compiler calls it (order 4),
+// and the call's stack frame lives in scratch memory.
 template <int ND, int ORDER, bool DEPOSIT>
-__device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ specs, int ns, const GatherFields &F,
-                                  const JPtrs &J, int *__restrict__ err, int cluster)
+__device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ specs,
+                                                                 int ns, const GatherFields &F, const JPtrs &J,
+                                                                 int *__restrict__ err, int cluster)
 {
     using TL = Tile<ORDER>;
     constexpr int H = TL::H, GT = TL::GT, JT = TL::JT, S = TL::S;
@@ -78,9 +81,11 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
     // 3D deposit: particles whose node changes on some axis are deposited in
     // a second pass over this queue (old delta, new position in cells, charge
     // weight, old node relative to the cluster), so that the main loop runs
-    // the narrow stencil with all lanes
+    // the narrow stencil with all lanes.  Order 4: a short queue, so that the
+    // workgroup's LDS (field and J tiles of 10^3 and 11^3 points) stays within
+    // 80 KB and two workgroups fit a CU.
     constexpr bool QUEUE = DEPOSIT && ND == 3;
-    constexpr int QCAP = QUEUE ? 128 : 1;
+    constexpr int QCAP = QUEUE ? (ORDER == 4 ? 12 : 128) : 1;
     __shared__ double sQ[QCAP][8];
     __shared__ int sQn;
 
@@ -366,7 +371,7 @@ __device__ inline void fp_cluster(const Geom &g, const FpSpecies *__restrict__ s
 
 // One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
 template <int ND, int ORDER, bool DEPOSIT>
-__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
+__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, ND == 3 && ORDER == 4 ? 2 : 3)
 k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err,
             int xcd)
 {
@@ -377,7 +382,7 @@ k_particles(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F,
 // from the work list (the non-empty clusters, wcount[0] of them) through the
 // queue head wcount[1]; every workgroup leaves once the list is exhausted.
 template <int ND, int ORDER, bool DEPOSIT>
-__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, 3)
+__global__ void __launch_bounds__(ND == 3 && ORDER == 4 ? 256 : 512, ND == 3 && ORDER == 4 ? 2 : 3)
 k_particles_list(Geom g, const FpSpecies *__restrict__ specs, int ns, GatherFields F, JPtrs J, int *__restrict__ err,
                  const int *__restrict__ work, int *__restrict__ wcount)
 {
