@@ -23,8 +23,10 @@ struct Dbl3 {
 };
 
 // Interpolator3D{2,4}Order::fieldsWrapper, one thread per particle.
+// (launched with 256 threads: the bound leaves the compiler 256 registers
+// instead of the 128 of a 1024-thread default, so nothing spills)
 template <int ORDER>
-__global__ void k_interp3d(const double *Ex, const double *Ey, const double *Ez, const double *Bx, const double *By,
+__global__ void __launch_bounds__(256, 2) k_interp3d(const double *Ex, const double *Ey, const double *Ez, const double *Bx, const double *By,
                            const double *Bz, Ints3 np, Ints3 dom, Dbl3 dinv, int64_t n, const double *x,
                            const double *y, const double *z, double *Ep, double *Bp, int32_t *iold, double *dold)
 {
@@ -156,8 +158,9 @@ __device__ inline bool proj3_particle_shapes(const Proj3Args &a, int64_t i, doub
 }
 
 // Projector3D{2,4}Order::currents with FP64 atomics, one thread per particle.
+// (launched with 128 threads; register budget as k_interp3d)
 template <int ORDER>
-__global__ void k_proj3d_atomic(Proj3Args a)
+__global__ void __launch_bounds__(128, 2) k_proj3d_atomic(Proj3Args a)
 {
     constexpr int H = ORDER / 2;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
