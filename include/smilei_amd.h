@@ -90,6 +90,10 @@ typedef struct {
 } smi_params;
 
 const char *smi_last_error(void);
+/* Hash of the sources the library was built from (16 hex digits); no
+ * counterpart in the reference, whose build has none.  The Python loader
+ * refuses a library whose hash differs from the sources beside it. */
+const char *smi_build_hash(void);
 
 /* ---- context API --------------------------------------------------- */
 int smi_create(const smi_params *params, smi_context **out);

@@ -53,6 +53,9 @@ int     ora_get_field(void *h, int fid, double *global);
 int     ora_step(void *h, int nsteps);
 /* out[0] = Uelm, out[1+s] = Ukin of species s */
 int     ora_energies(void *h, double *out);
+/* out[0..5] = Uelm of Ex, Ey, Ez, Bx_m, By_m, Bz_m: 0.5 * V_cell * the
+ * Field3D::norm2 sums over each patch's istart/bufsize ranges. */
+int     ora_field_energies(void *h, double *out);
 /* Charge density of all species (order-matched shape, reference
  * Projector*::basic type 0), ghost-summed, as a global primal array. */
 int     ora_rho(void *h, double *global);

@@ -569,25 +569,79 @@ int ora_timers(void *h, double *out)
     return 0;
 }
 
-// DiagnosticScalar (DiagnosticScalar.cpp:494-522, 681): Ukin_s = m_s *
-// sum w (gamma - 1); Uelm = 0.5 * V_cell * sum over the patch interiors of
-// Ex^2+Ey^2+Ez^2+Bx_m^2+By_m^2+Bz_m^2.
-int ora_energies(void *h, double *out)
+}  // extern "C"
+
+namespace {
+
+// Local index range [lo, lo+n) of a patch array along axis d that
+// Field3D::norm2 sums (istart / bufsize of ElectroMagn3D.cpp:185-226): the
+// "non duplicated elements", which with periodic boundaries still include
+// the shared face point of the first patch (primal) and one face point at
+// each end of the domain (dual), i.e. N+1 primal and N+2 dual points per
+// axis over all patches.
+void norm2_range(const Sim &S, const OPatch &P, int d, int dual, int &lo, int &n)
+{
+    if (d >= S.nd) {
+        lo = 0;
+        n = 1;
+        return;
+    }
+    lo = S.os + (P.pc[d] != 0 ? 1 : 0);
+    n = S.p.patch_size[d] + 1 + dual;
+    if (S.npatch[d] != 1) {
+        if (!dual && P.pc[d] != 0) {
+            n -= 1;
+        } else if (dual) {
+            n -= 1;
+            if (P.pc[d] != 0 && P.pc[d] != S.npatch[d] - 1) n -= 1;
+        }
+    }
+}
+
+// Field3D::norm2 (Field3D.cpp:230-250) of component f summed over patches.
+double field_norm2(const Sim &S, int f)
+{
+    double a = 0.;
+    for (const auto &P : S.patches) {
+        int lo[3], n[3];
+        for (int d = 0; d < 3; ++d) norm2_range(S, P, d, kDual[f][d], lo[d], n[d]);
+        const int *dm = S.dims[f];
+        for (int i = lo[0]; i < lo[0] + n[0]; ++i)
+            for (int j = lo[1]; j < lo[1] + n[1]; ++j)
+                for (int k = lo[2]; k < lo[2] + n[2]; ++k) {
+                    const double v = P.F[f][((size_t)i * dm[1] + j) * dm[2] + k];
+                    a += v * v;
+                }
+    }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Per-field electromagnetic energies of DiagnosticScalar::compute
+// (DiagnosticScalar.cpp:660-690): Uelm_f = 0.5 * V_cell * norm2(f) for
+// Ex, Ey, Ez, Bx_m, By_m, Bz_m.
+int ora_field_energies(void *h, double *out)
 {
     Sim *S = (Sim *)h;
     double cell_volume = 1.0;
     for (int d = 0; d < S->nd; ++d) cell_volume *= S->p.cell_length[d];
-    const size_t NN = (size_t)S->p.n_cells[0] * S->p.n_cells[1] * S->p.n_cells[2];
-    std::vector<double> g(NN);
-    double uelm = 0.;
     const int fl[6] = {EX, EY, EZ, BXM, BYM, BZM};
-    for (int f : fl) {
-        ora_get_field(h, f, g.data());
-        double a = 0.;
-        for (size_t i = 0; i < NN; ++i) a += g[i] * g[i];
-        uelm += a;
-    }
-    out[0] = 0.5 * cell_volume * uelm;
+    for (int q = 0; q < 6; ++q) out[q] = 0.5 * cell_volume * field_norm2(*S, fl[q]);
+    return 0;
+}
+
+// DiagnosticScalar (DiagnosticScalar.cpp:494-522, 660-690): Ukin_s = m_s *
+// sum w (gamma - 1); Uelm = sum of the six per-field energies
+// (ElectroMagn::computeEnergy, ElectroMagn.cpp:465-477).
+int ora_energies(void *h, double *out)
+{
+    Sim *S = (Sim *)h;
+    double fe[6];
+    ora_field_energies(h, fe);
+    out[0] = fe[0] + fe[1] + fe[2] + fe[3] + fe[4] + fe[5];
     for (size_t s = 0; s < S->species.size(); ++s) {
         double e = 0.;
         for (auto &P : S->patches) {

@@ -57,6 +57,7 @@ def lib():
         L.ora_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         L.ora_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ora_energies.argtypes = [ctypes.c_void_p, _D]
+        L.ora_field_energies.argtypes = [ctypes.c_void_p, _D]
         L.ora_rho.argtypes = [ctypes.c_void_p, _D]
         L.ora_gauss.argtypes = [ctypes.c_void_p, _D]
         L.ora_timers.argtypes = [ctypes.c_void_p, _D]
@@ -159,6 +160,11 @@ class OracleSim:
     def energies(self):
         out = np.empty(1 + self.nspecies)
         lib().ora_energies(self.h, dptr(out))
+        return out
+
+    def field_energies(self):
+        out = np.empty(6)
+        lib().ora_field_energies(self.h, dptr(out))
         return out
 
     def rho(self):

@@ -1,10 +1,13 @@
 """ctypes loader of libsmilei_amd.so (the HIP library built from csrc/).
 
-The library is required: there is no CPU fallback.  Loading fails with an
-ImportError-like RuntimeError if it has not been built.
+The library is required: there is no CPU fallback.  Loading fails with a
+RuntimeError if it has not been built, or if it was built from sources other
+than the ones beside it (the build embeds a hash of csrc/ and the header).
 """
 import ctypes
 import os
+
+from . import srchash
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsmilei_amd.so")
@@ -38,6 +41,7 @@ class SmiParams(ctypes.Structure):
 # Every symbol include/smilei_amd.h declares, with its ctypes signature.
 SIGNATURES = {
     "smi_last_error": (ctypes.c_char_p, []),
+    "smi_build_hash": (ctypes.c_char_p, []),
     "smi_create": (ctypes.c_int, [ctypes.POINTER(SmiParams), ctypes.POINTER(_VP)]),
     "smi_destroy": (ctypes.c_int, [_VP]),
     "smi_add_species": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
@@ -88,6 +92,25 @@ SIGNATURES = {
 }
 
 _lib = None
+_TAG = b"smilei-amd-source-hash:"
+
+
+def built_hash(path=LIB_PATH):
+    """Source hash embedded in the built library (read from the file, so a
+    stale library is found before it is loaded), or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path, "rb") as f:
+        data = f.read()
+    i = data.find(_TAG)
+    if i < 0:
+        return None
+    return data[i + len(_TAG):i + len(_TAG) + 16].decode("ascii", "replace")
+
+
+def stale():
+    """True if the library is missing or was built from other sources."""
+    return built_hash() != srchash.source_hash()
 
 
 def load():
@@ -98,6 +121,10 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             "libsmilei_amd.so is not built (%s); run __graft_entry__.build() or make -C smilei_amd/csrc" % LIB_PATH)
+    have, want = built_hash(), srchash.source_hash()
+    if have != want:
+        raise RuntimeError("libsmilei_amd.so was built from other sources (library %s, tree %s); rebuild it with "
+                           "__graft_entry__.build() or make -C smilei_amd/csrc" % (have, want))
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

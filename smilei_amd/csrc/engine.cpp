@@ -566,6 +566,9 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
     SpeciesState &S = c->sp[s];
     if (n < 0 || n > S.cap) throw Error(SMI_ERR_CAPACITY, "particle count exceeds the species capacity");
     const double *src[7] = {x, y, z, px, py, pz, w};
+    // z may be NULL in 2D; every other array is required when n > 0
+    for (int q = 0; q < 7; ++q)
+        if (n > 0 && !src[q] && !(q == 2 && c->g.nd < 3)) throw Error(SMI_ERR_INVALID, "null particle array");
     for (int d = 0; d < c->g.nd; ++d) {
         const double *v = src[d];
         for (int64_t i = 0; i < n; ++i)
@@ -580,17 +583,16 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
         double wmax = 0.;
         for (int64_t i = 0; i < n; ++i) wmax = std::max(wmax, std::fabs(w[i]));
         set_quantum(c->g, S, wmax);
-        // stage FP64 on the device, convert to cell + FP32 offset
-        std::vector<double *> tmp(7, nullptr);
+        // stage FP64 on the device, convert to cell + FP32 offset (the
+        // staging buffers are freed on every exit path)
+        DevBuf<double> tmp[7];
         for (int q = 0; q < 7; ++q) {
-            tmp[q] = dalloc<double>(n);
-            if (!src[q]) throw Error(SMI_ERR_INVALID, "null particle array");
-            if (n) SMI_HIP(hipMemcpyAsync(tmp[q], src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
+            tmp[q].alloc(n);
+            if (n) SMI_HIP(hipMemcpyAsync(tmp[q].p, src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
         }
-        launch_to_mixed(c->g, PartPtrs{tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]}, n, S.mview(c->cur),
-                        S.keys[c->cur], c->st);
+        launch_to_mixed(c->g, PartPtrs{tmp[0].p, tmp[1].p, tmp[2].p, tmp[3].p, tmp[4].p, tmp[5].p, tmp[6].p}, n,
+                        S.mview(c->cur), S.keys[c->cur], c->st);
         SMI_HIP(hipStreamSynchronize(c->st));
-        for (auto t : tmp) (void)hipFree(t);
     } else {
         for (int q = 0; q < 7; ++q) {
             double *dst = (double *)S.buf[c->cur][q];
@@ -598,7 +600,6 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
                 launch_fill(dst, n, 0.0, c->st);
                 continue;
             }
-            if (!src[q]) throw Error(SMI_ERR_INVALID, "null particle array");
             if (n) SMI_HIP(hipMemcpyAsync(dst, src[q], n * sizeof(double), hipMemcpyHostToDevice, c->st));
         }
         launch_bin_initial(c->g, S.view(c->cur), n, S.keys[c->cur], c->st);
@@ -1045,9 +1046,17 @@ int smi_step(smi_context *c, int nsteps)
             // graph[k]: one step reading particle buffer k (captured once)
             const int k = c->cur;
             if (!c->graph[k]) {
-                hipGraph_t gr;
+                hipGraph_t gr = nullptr;
                 SMI_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-                enqueue_step(c, false);
+                try {
+                    enqueue_step(c, false);
+                } catch (...) {
+                    // leave the stream out of capture mode, drop the partial graph
+                    hipGraph_t partial = nullptr;
+                    (void)hipStreamEndCapture(c->st, &partial);
+                    if (partial) (void)hipGraphDestroy(partial);
+                    throw;
+                }
                 SMI_HIP(hipStreamEndCapture(c->st, &gr));
                 SMI_HIP(hipGraphInstantiate(&c->graph[k], gr, nullptr, nullptr, 0));
                 SMI_HIP(hipGraphDestroy(gr));
@@ -1245,7 +1254,7 @@ int smi_scalars(smi_context *c, double *out)
     const int fid[6] = {SMI_EX, SMI_EY, SMI_EZ, SMI_BXM, SMI_BYM, SMI_BZM};
     for (int f = 0; f < 6; ++f) {
         const double *one[1] = {c->F[fid[f]]};
-        launch_sumsq(c->g, one, 1, c->partials, kReduceBlocks, c->st);
+        launch_sumsq(c->g, one, &fid[f], 1, c->partials, kReduceBlocks, c->st);
         out[f] = 0.5 * cell_volume * reduce_partials(c);
     }
     for (size_t k = 0; k < c->sp.size(); ++k) {
@@ -1269,8 +1278,9 @@ int smi_energies(smi_context *c, double *out)
     check_ctx(c);
     SMI_HIP(hipSetDevice(c->p.device));
     std::vector<double> h(kReduceBlocks);
+    const int fid[6] = {SMI_EX, SMI_EY, SMI_EZ, SMI_BXM, SMI_BYM, SMI_BZM};
     const double *fl[6] = {c->F[SMI_EX], c->F[SMI_EY], c->F[SMI_EZ], c->F[SMI_BXM], c->F[SMI_BYM], c->F[SMI_BZM]};
-    launch_sumsq(c->g, fl, 6, c->partials, kReduceBlocks, c->st);
+    launch_sumsq(c->g, fl, fid, 6, c->partials, kReduceBlocks, c->st);
     SMI_HIP(hipMemcpyAsync(h.data(), c->partials, kReduceBlocks * sizeof(double), hipMemcpyDeviceToHost, c->st));
     SMI_HIP(hipStreamSynchronize(c->st));
     double s = 0;

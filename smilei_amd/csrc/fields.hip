@@ -144,18 +144,32 @@ __device__ inline double block_sum(double v)
 
 struct FieldList {
     const double *f[6];
+    int dual[6];  // staggering bits, axis d at bit d
     int nf;
 };
 
+// Sum of squares with the point multiplicities of Field3D::norm2 over the
+// patches' istart/bufsize ranges (ElectroMagn3D.cpp:185-226,
+// Field3D.cpp:230-250): along each axis of the simulation those ranges cover
+// N+1 primal points (global 0..N, N being the periodic image of 0) and N+2
+// dual points (0..N+1, images of 0 and 1), so in the periodic global array
+// primal index 0 and dual indices 0 and 1 count twice.
 __global__ void k_sumsq(Geom g, FieldList L, double *partials)
 {
     double s = 0.;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.ncells;
-         t += (int64_t)gridDim.x * blockDim.x)
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int idx[3] = {(int)(t / ((int64_t)g.N[1] * g.N[2])), (int)((t / g.N[2]) % g.N[1]), (int)(t % g.N[2])};
         for (int c = 0; c < L.nf; ++c) {
+            double m = 1.;
+            for (int d = 0; d < g.nd; ++d) {
+                const bool dual = (L.dual[c] >> d) & 1;
+                if (idx[d] == 0 || (dual && idx[d] == 1)) m *= 2.;
+            }
             const double v = L.f[c][t];
-            s += v * v;
+            s += m * (v * v);
         }
+    }
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -259,11 +273,19 @@ void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st)
     SMI_LAUNCH_CHECK();
 }
 
-void launch_sumsq(const Geom &g, const double *const *fields, int nf, double *partials, int nblocks, hipStream_t st)
+void launch_sumsq(const Geom &g, const double *const *fields, const int *fids, int nf, double *partials,
+                  int nblocks, hipStream_t st)
 {
+    // staggering of Ex Ey Ez Bx By Bz Bx_m By_m Bz_m Jx Jy Jz (bit d: dual along d)
+    static const int kDualBits[12] = {1, 2, 4, 6, 5, 3, 6, 5, 3, 1, 2, 4};
+    if (nf < 1 || nf > 6) throw Error(-1, "sumsq takes 1 to 6 fields");
     FieldList L;
     L.nf = nf;
-    for (int c = 0; c < nf; ++c) L.f[c] = fields[c];
+    for (int c = 0; c < nf; ++c) {
+        if (fids[c] < 0 || fids[c] >= 12) throw Error(-1, "bad field id");
+        L.f[c] = fields[c];
+        L.dual[c] = kDualBits[fids[c]];
+    }
     hipLaunchKernelGGL(k_sumsq, dim3(nblocks), dim3(256), 0, st, g, L, partials);
     SMI_LAUNCH_CHECK();
 }

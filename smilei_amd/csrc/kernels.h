@@ -227,8 +227,9 @@ void launch_maxwell(const Geom &g, const FieldSet &f, hipStream_t st);
 void launch_fill(double *p, int64_t n, double v, hipStream_t st);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 // Diagnostics.
-void launch_sumsq(const Geom &g, const double *const *fields, int nf, double *partials, int nblocks,
-                  hipStream_t st);
+// fids: the SMI_* component ids of the fields (their staggering).
+void launch_sumsq(const Geom &g, const double *const *fields, const int *fids, int nf, double *partials,
+                  int nblocks, hipStream_t st);
 void launch_ukin(const PartPtrs &A, const int32_t *perm, int64_t n, double *partials, int nblocks,
                  hipStream_t st);
 // Sum of n weights (FP64 w or FP32 wf), per-block partials.

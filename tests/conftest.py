@@ -13,8 +13,12 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session", autouse=True)
-def _oracle_built():
-    """The oracle library is the checker of every test; build it if absent."""
+def _libraries_built():
+    """The oracle library is the checker of every test: build it if absent.
+    The HIP library is rebuilt if it was built from other sources (the
+    loader refuses such a library)."""
     from oracle import pyoracle
     if not os.path.exists(pyoracle.LIB_PATH):
         pyoracle.build()
+    import __graft_entry__
+    __graft_entry__.build_if_stale()

@@ -336,6 +336,73 @@ def test_yee_exact_discrete_plane_wave(axis):
                                                                  np.cos(k * xb - w * (n - 0.5) * dt)), atol=5e-13)
 
 
+def _yee2d_mode(family, N, dx, m, dt):
+    """Exact discrete plane wave of the 2D Yee scheme with wave vector
+    k = 2 pi (m0 / L0, m1 / L1).  The Yee differences turn k into
+    K_d = 2 sin(k_d dx_d / 2) / dx_d and w into W = 2 sin(w dt / 2) / dt, so
+    the mode obeys W^2 = K0^2 + K1^2.  family "Ez": Ez = cos(phi) with
+    Bx = (K1/W) cos, By = -(K0/W) cos; family "Bz": Bz = cos(phi) with
+    Ex = -(K1/W) cos, Ey = (K0/W) cos (signs from dBx/dt = -dEz/dy,
+    dBy/dt = dEz/dx, dEx/dt = dBz/dy, dEy/dt = -dBz/dx).  Returns
+    f(name, t) giving each component at its staggered points at time t."""
+    k = [2 * np.pi * m[d] / (N[d] * dx[d]) for d in range(2)]
+    K = [2 * np.sin(k[d] * dx[d] / 2) / dx[d] for d in range(2)]
+    W = np.sqrt(K[0] ** 2 + K[1] ** 2)
+    w = 2 / dt * np.arcsin(W * dt / 2)
+    dual = {"Ex": (1, 0), "Ey": (0, 1), "Ez": (0, 0), "Bx": (0, 1), "By": (1, 0), "Bz": (1, 1)}
+    amp = ({"Ez": 1.0, "Bx": K[1] / W, "By": -K[0] / W} if family == "Ez" else
+           {"Bz": 1.0, "Ex": -K[1] / W, "Ey": K[0] / W})
+    idx = np.indices(tuple(N)).astype(float)
+
+    def f(name, t):
+        if name not in amp:
+            return np.zeros(tuple(N))
+        pos = [(idx[d] - 0.5 * dual[name][d]) * dx[d] for d in range(2)]
+        return amp[name] * np.cos(k[0] * pos[0] + k[1] * pos[1] - w * t)
+    return f
+
+
+@pytest.mark.parametrize("family", ["Ez", "Bz"])
+@pytest.mark.parametrize("m", [(2, 0), (0, 1), (1, 2)])
+def test_yee2d_exact_discrete_plane_wave(family, m):
+    """2D Yee (MA_Solver2D_norm, MF_Solver2D_Yee): both polarisations,
+    propagating along x, along y and obliquely, on cells with dx != dy, are
+    exact eigenmodes; a swapped dt/dx, dt/dy or a flipped curl sign changes
+    the dispersion or the amplitudes and fails at O(1)."""
+    N, dx = (16, 12), (0.125, 0.2)
+    dt = 0.95 * cfl_dt(dx)
+    S = O.OracleSim(2, 2, N, (8, 4), dx, dt)
+    f = _yee2d_mode(family, N, dx, m, dt)
+    for name in ("Ex", "Ey", "Ez"):
+        S.set_field(name, f(name, 0.0))
+    for name in ("Bx", "By", "Bz"):
+        S.set_field(name, f(name, 0.5 * dt))
+    n = 30
+    S.step(n)
+    for name in ("Ex", "Ey", "Ez"):
+        np.testing.assert_allclose(S.get_field(name).reshape(N), f(name, n * dt), atol=5e-13, err_msg=name)
+    for name in ("Bx", "By", "Bz"):
+        np.testing.assert_allclose(S.get_field(name).reshape(N), f(name, (n + 0.5) * dt), atol=5e-13, err_msg=name)
+        np.testing.assert_allclose(S.get_field(name + "_m").reshape(N),
+                                   0.5 * (f(name, (n + 0.5) * dt) + f(name, (n - 0.5) * dt)), atol=5e-13)
+
+
+def test_yee2d_preserves_div_b():
+    rng = np.random.default_rng(19)
+    N, dx = (16, 8), (0.125, 0.2)
+    S = O.OracleSim(2, 2, N, (8, 4), dx, 0.95 * cfl_dt(dx))
+    for k in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
+        S.set_field(k, rng.standard_normal(N))
+
+    def divB():
+        Bx, By = S.get_field("Bx").reshape(N), S.get_field("By").reshape(N)
+        return (Bx - np.roll(Bx, 1, 0)) / dx[0] + (By - np.roll(By, 1, 1)) / dx[1]
+
+    d0 = divB()
+    S.step(10)
+    np.testing.assert_allclose(divB(), d0, atol=1e-11)
+
+
 def test_yee_preserves_div_b():
     rng = np.random.default_rng(9)
     shape = (16, 8, 16)
@@ -386,6 +453,56 @@ def _patch_of(pos, dx, shape, patch, nd):
     return (pc[0] * np_[1] + pc[1]) * np_[2] + pc[2]
 
 
+def _key_table(patch, nd):
+    """In-patch cell key of every cell by enumeration, in the order the
+    reference's Cluster3D::CellKey (nvidiaParticles.cu:639-660) ranks cells:
+    clusters of 4 cells per axis walked x-major (x, then y, then z), and
+    inside a cluster its cells walked x-major.  The table is built by
+    counting, not by the key's arithmetic, so a transposed or mis-scaled
+    key formula in the oracle disagrees with it."""
+    ext = list(patch[:nd]) + [1] * (3 - nd)
+    w = [4 if d < nd else 1 for d in range(3)]
+    table = np.full(ext, -1, np.int64)
+    n = 0
+    for cx in range(ext[0] // w[0]):
+        for cy in range(ext[1] // w[1]):
+            for cz in range(ext[2] // w[2]):
+                for ix in range(w[0]):
+                    for iy in range(w[1]):
+                        for iz in range(w[2]):
+                            table[cx * w[0] + ix, cy * w[1] + iy, cz * w[2] + iz] = n
+                            n += 1
+    assert n == np.prod(ext) and table.min() == 0
+    return table
+
+
+def _expected_keys(p, dx, shape, patch, nd):
+    """Keys of the cells the positions lie in, looked up in _key_table."""
+    npc = [shape[d] // patch[d] if d < nd else 1 for d in range(3)]
+    pid = p["patch"].astype(np.int64)
+    pc = [pid // (npc[1] * npc[2]), (pid // npc[2]) % npc[1], pid % npc[2]]
+    pos = [p["x"], p["y"], p["z"]]
+    ic = []
+    for d in range(3):
+        if d < nd:
+            c = np.floor(pos[d] * (1.0 / dx[d])).astype(np.int64) - pc[d] * patch[d]
+            ic.append(np.clip(c, 0, patch[d] - 1))
+        else:
+            ic.append(np.zeros_like(pid))
+    return _key_table(patch, nd)[ic[0], ic[1], ic[2]]
+
+
+def test_key_table_matches_reference_examples():
+    """Hand-checked keys of a 8x8x8 patch (2x2x2 clusters): cell (0,0,1) is
+    the second cell of cluster 0, (1,0,0) the 17th, (0,0,4) starts the
+    second cluster (z is the fastest cluster axis), (4,0,0) the fifth."""
+    t = _key_table((8, 8, 8), 3)
+    assert t[0, 0, 1] == 1 and t[0, 1, 0] == 4 and t[1, 0, 0] == 16
+    assert t[0, 0, 4] == 64 and t[0, 4, 0] == 128 and t[4, 0, 0] == 256 and t[7, 7, 7] == 511
+    t2 = _key_table((8, 12), 2)
+    assert t2[0, 1, 0] == 1 and t2[1, 0, 0] == 4 and t2[0, 4, 0] == 16 and t2[4, 0, 0] == 48
+
+
 @pytest.mark.parametrize("nd,order,shape,patch", [
     (3, 2, (16, 16, 16), (8, 8, 8)),
     (3, 4, (24, 12, 12), (12, 12, 12)),
@@ -406,6 +523,8 @@ def test_step_invariants(nd, order, shape, patch):
         # owner patch is the one containing the position
         pos = [p["x"], p["y"], p["z"]]
         np.testing.assert_array_equal(p["patch"], _patch_of(pos, dx, shape, patch, nd))
+        # every key is the enumerated key of the cell holding the position
+        np.testing.assert_array_equal(p["key"], _expected_keys(p, dx, shape, patch, nd))
         # keys sorted inside each patch and counts equal to the histogram
         for q in range(S.npatch):
             kq = p["key"][p["patch"] == q]
@@ -443,6 +562,74 @@ def test_decomposition_invariance(nd, order, shape, patches):
             scale = np.abs(results[0][k]).max()
             assert np.abs(r[k] - results[0][k]).max() <= 1e-12 * scale, k
         np.testing.assert_allclose(r["parts"], results[0]["parts"], rtol=1e-12, atol=1e-14)
+
+
+_DUAL = {"Ex": (1, 0, 0), "Ey": (0, 1, 0), "Ez": (0, 0, 1), "Bx_m": (0, 1, 1), "By_m": (1, 0, 1), "Bz_m": (1, 1, 0)}
+_EFIELDS = ("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m")
+
+
+def _norm2_extended(a, dual, nd):
+    """Sum of squares over global indices 0..N (primal) or 0..N+1 (dual)
+    per axis, read periodically: the points the reference's norm2 visits
+    over all patches (ElectroMagn3D.cpp:185-226)."""
+    idx = [np.arange(a.shape[d] + 1 + dual[d]) % a.shape[d] if d < nd else np.arange(a.shape[d])
+           for d in range(a.ndim)]
+    return float((a[np.ix_(*idx)] ** 2).sum())
+
+
+@pytest.mark.parametrize("nd,shape,patch", [
+    (3, (16, 8, 12), (8, 4, 4)),   # several patches per axis
+    (3, (8, 8, 8), (8, 4, 8)),     # one patch along x and z
+    (2, (16, 12), (8, 4)),
+])
+def test_field_energy_closed_form(nd, shape, patch):
+    """Uelm_f = 0.5 V_cell sum f^2 over the istart/bufsize ranges: a uniform
+    field c_f gives 0.5 V c_f^2 prod_d (N_d + 1 + dual_d); a random periodic
+    field the sum over the periodically extended index ranges.  B_m (not B)
+    enters; no ghost point beyond the ranges, no interior point twice."""
+    dx = (0.3, 0.2, 0.25)[:nd]
+    S = O.OracleSim(nd, 2, shape, patch, dx, 0.5 * cfl_dt(dx))
+    V = float(np.prod(dx))
+    full = tuple(shape) + (1,) * (3 - nd)
+    for i, k in enumerate(_EFIELDS):
+        S.set_field(k, np.full(full, 1.0 + i))
+    for k in ("Bx", "By", "Bz"):
+        S.set_field(k, np.full(full, 100.0))  # must not enter
+    fe = S.field_energies()
+    for i, k in enumerate(_EFIELDS):
+        pts = np.prod([shape[d] + 1 + _DUAL[k][d] for d in range(nd)])
+        np.testing.assert_allclose(fe[i], 0.5 * V * (1.0 + i) ** 2 * pts, rtol=1e-14, err_msg=k)
+    rng = np.random.default_rng(4)
+    arrs = {k: rng.standard_normal(full) for k in _EFIELDS}
+    for k in _EFIELDS:
+        S.set_field(k, arrs[k])
+    fe = S.field_energies()
+    for i, k in enumerate(_EFIELDS):
+        np.testing.assert_allclose(fe[i], 0.5 * V * _norm2_extended(arrs[k], _DUAL[k], nd), rtol=1e-13,
+                                   err_msg=k)
+    np.testing.assert_allclose(S.energies()[0], fe.sum(), rtol=1e-15)
+
+
+def test_kinetic_energy_closed_form():
+    """Ukin_s = m_s sum w (gamma - 1) for particles of known momenta."""
+    shape, patch, dx = (8, 8, 8), (4, 4, 4), (0.2, 0.2, 0.2)
+    S = O.OracleSim(3, 2, shape, patch, dx, 0.5 * cfl_dt(dx))
+    masses = (1.0, 1836.0)
+    u = [np.array([[3.0, 0.0, 4.0], [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]]),
+         np.array([[0.0, 1.0, 0.0], [2.0, 2.0, 1.0]])]
+    w = [np.array([0.5, 2.0, 1.5]), np.array([0.25, 4.0])]
+    # gamma = sqrt(1 + |u|^2): 26**.5, 1, 2; 2**.5, sqrt(10)
+    expect = [1.0 * (0.5 * (26 ** 0.5 - 1) + 0 + 1.5 * (2 - 1)),
+              1836.0 * (0.25 * (2 ** 0.5 - 1) + 4.0 * (10 ** 0.5 - 1))]
+    rng = np.random.default_rng(1)
+    for s in range(2):
+        sid = S.add_species(masses[s], -1.0 if s == 0 else 1.0)
+        n = len(w[s])
+        pos = [rng.random(n) * shape[d] * dx[d] for d in range(3)]
+        S.set_particles(sid, *pos, u[s][:, 0], u[s][:, 1], u[s][:, 2], w[s])
+    e = S.energies()
+    assert e[0] == 0.0
+    np.testing.assert_allclose(e[1:], expect, rtol=1e-14)
 
 
 def test_thermal_energy_history_short():
