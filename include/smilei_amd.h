@@ -30,6 +30,9 @@
 #define SMILEI_AMD_H
 
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -274,6 +277,100 @@ int smi_maxwell_ampere_3d(double *d_Ex, double *d_Ey, double *d_Ez,
 int smi_maxwell_faraday_3d(double *d_Bx, double *d_By, double *d_Bz,
                            const double *d_Ex, const double *d_Ey, const double *d_Ez,
                            const int np[3], double timestep, const double cell_length[3], void *stream);
+
+/* Projector3D2Order / Projector3D4Order ::basic, type 0 (charge density at
+ * the particles' positions: diagnostic steps of currentsAndDensityWrapper
+ * with diag_flag, frozen species; Projector3D2Order.cpp:526-621) over
+ * particles [0,n) of one species, into the patch's primal rho array (or
+ * rho_s of the species).  FP64 atomics: equal to the CPU operator up to the
+ * order of the sums. */
+int smi_density_3d(int order, double *d_rho, const int np[3], const int dom_begin[3], const double cell_length[3],
+                   double charge, int64_t n, const double *d_x, const double *d_y, const double *d_z,
+                   const double *d_w, void *stream);
+/* Projector2D2Order::basic, type 0 (Projector2D2Order.cpp). */
+int smi_density_2d(int order, double *d_rho, const int np[2], const int dom_begin[2], const double cell_length[2],
+                   double charge, int64_t n, const double *d_x, const double *d_y, const double *d_w, void *stream);
+/* MA_Solver2D_norm / MF_Solver2D_Yee ::operator()
+ * (src/ElectroMagnSolver/MA_Solver2D_norm.cpp:18-105,
+ *  src/ElectroMagnSolver/MF_Solver2D_Yee.cpp:18-99) on one patch. */
+int smi_maxwell_ampere_2d(double *d_Ex, double *d_Ey, double *d_Ez,
+                          const double *d_Bx, const double *d_By, const double *d_Bz,
+                          const double *d_Jx, const double *d_Jy, const double *d_Jz,
+                          const int np[2], double timestep, const double cell_length[2], void *stream);
+int smi_maxwell_faraday_2d(double *d_Bx, double *d_By, double *d_Bz,
+                           const double *d_Ex, const double *d_Ey, const double *d_Ez,
+                           const int np[2], double timestep, const double cell_length[2], void *stream);
+
+/* ---- the reference's own C seams, same argument lists ---------------- *
+ * currentDeposition3DOnDevice / densityDeposition3DOnDevice
+ * (src/Projector/Projector3D2OrderGPU.cpp:55-120, called from
+ * Projector3D2OrderGPU::currentsAndDensityWrapper :380-487).  Every pointer
+ * is a device pointer (the reference's launcher maps its host pointers with
+ * HostDeviceMemoryManagement::GetDevicePointer before its kernel launch; a
+ * binding passes the mapped pointers here).  The particles of one patch are
+ * binned by 4^3 cluster in the x-major cluster order: bin_index[b] is the
+ * end of bin b (Particles::last_index), x/y/z_dimension_bin_count the
+ * clusters per axis, iold / deltaold the SmileiMPI scratch of the
+ * interpolator with stride number_of_particles, not_spectral = 1 for the
+ * staggered Jy / Jz of the Yee solver, cell_sorting selects the loop order
+ * of a bin (results differ only by the order of the atomic sums).  Order-2
+ * shape, as the reference's GPU kernels.  The work runs on the null stream
+ * and has finished when the call returns, as in the reference (which
+ * synchronises the device after its kernel).  Returns SMI_ERR_INVALID, and
+ * skips the particle, when a particle's stencil leaves its bin's tile. */
+int smi_current_deposition_3d(double *Jx, double *Jy, double *Jz, int Jx_size, int Jy_size, int Jz_size,
+                              const double *particle_position_x, const double *particle_position_y,
+                              const double *particle_position_z, const short *particle_charge,
+                              const double *particle_weight, const int *bin_index,
+                              unsigned int x_dimension_bin_count, unsigned int y_dimension_bin_count,
+                              unsigned int z_dimension_bin_count, const double *invgf, const int *iold,
+                              const double *deltaold, unsigned int number_of_particles, double inv_cell_volume,
+                              double dx_inv, double dy_inv, double dz_inv, double dx_ov_dt, double dy_ov_dt,
+                              double dz_ov_dt, int i_domain_begin, int j_domain_begin, int k_domain_begin,
+                              int nprimy, int nprimz, int not_spectral, bool cell_sorting);
+int smi_density_deposition_3d(double *rho, int rho_size, const double *particle_position_x,
+                              const double *particle_position_y, const double *particle_position_z,
+                              const short *particle_charge, const double *particle_weight, const int *bin_index,
+                              unsigned int x_dimension_bin_count, unsigned int y_dimension_bin_count,
+                              unsigned int z_dimension_bin_count, const double *invgf, const int *iold,
+                              const double *deltaold, unsigned int number_of_particles, double inv_cell_volume,
+                              double dx_inv, double dy_inv, double dz_inv, double dx_ov_dt, double dy_ov_dt,
+                              double dz_ov_dt, int i_domain_begin, int j_domain_begin, int k_domain_begin,
+                              int nprimy, int nprimz, int not_spectral, bool cell_sorting);
+
+/* ---- synchronisation between the patches of one device -------------- *
+ * The patches' ghosted arrays are held in one patch-major array: patch p
+ * (p = (px*npy+py)*npz+pz) owns the block [p*B, (p+1)*B) holding its three
+ * components one after the other, each in the per-patch layout
+ * (np = patch_size + 2*order + 1 points per axis, +1 along a dual axis).
+ * The domain is periodic. */
+/* SyncVectorPatch::sumAllComponents for Jx, Jy, Jz (src/Patch/
+ * SyncVectorPatch.cpp:203-~630): along x, then y, then z, the overlapping
+ * 2*order+1(+1 dual) planes of neighbours are summed and copied back. */
+int smi_sum_j_ghosts(int ndim, int order, const int n_cells[3], const int patch_size[3], double *d_J,
+                     void *stream);
+/* SyncVectorPatch::exchangeB for Bx, By, Bz (SyncVectorPatch.cpp:667,
+ * 1441-1830): per axis, the ghost planes of the components dual along it
+ * are copied from the neighbours' interiors. */
+int smi_exchange_b(int ndim, int order, const int n_cells[3], const int patch_size[3], double *d_B,
+                   void *stream);
+/* Particle exchange between the patches and cell sort of one species on one
+ * device: Patch::exchParticles / cornersParticles / importAndSortParticles
+ * (src/Patch/Patch.cpp:535-819) and Species::sortParticles
+ * (src/Species/Species.cpp:1046).  Input: the species' n particles of all
+ * patches in any order, positions at most one box length outside the
+ * periodic box (wrapped as Patch::prepareParticles does).  Output, in
+ * place: the particles ordered by owning patch, then by in-patch cell key
+ * (Cluster3D::CellKey, src/Particles/nvidiaParticles.cu:639-660), stable
+ * inside a cell; d_key (may be NULL) the in-patch cell keys, d_patch_count
+ * (may be NULL) the particles per patch, d_bin_index (may be NULL) per patch
+ * the cumulative ends of its 4^3 (2D: 4^2) clusters relative to the patch's
+ * first particle (Particles::last_index).  charge may be NULL; z may be
+ * NULL in 2D.  Allocates its scratch; returns once the stream drained. */
+int smi_sort_exchange(int ndim, const int n_cells[3], const int patch_size[3], const double cell_length[3],
+                      int64_t n, double *d_x, double *d_y, double *d_z, double *d_px, double *d_py, double *d_pz,
+                      double *d_w, int16_t *d_charge, int32_t *d_key, int32_t *d_patch_count,
+                      int32_t *d_bin_index, void *stream);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,14 @@ int     ora_gauss(void *h, double *global);
  * deposit), [1] exchange+sort, [2] J ghost sum, [3] Maxwell + B sync. */
 int     ora_timers(void *h, double *out);
 
+/* ---- multi-patch synchronisation on patch-major flat arrays ----------- *
+ * Patch p's block starts at p * (|X| + |Y| + |Z|) and holds the three
+ * components in the per-patch ghosted layout (patch ids (px*npy+py)*npz+pz). */
+/* SyncVectorPatch::sumAllComponents of Jx, Jy, Jz (x, y, z passes). */
+int ora_sum_j_ghosts(const ora_params *p, double *J);
+/* SyncVectorPatch::exchangeB of Bx, By, Bz (x, y, z passes). */
+int ora_exchange_b(const ora_params *p, double *B);
+
 /* ---- single-patch operators (reference per-patch layout) -------------- *
  * Field arrays are the patch's ghosted arrays (primal dims np = patch +
  * 2*order + 1, dual = np + 1 along the staggered axis), dom_begin is the

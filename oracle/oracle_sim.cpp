@@ -249,7 +249,9 @@ void exchange_and_sort(Sim &S, int s)
 // SyncVectorPatch::sumAllComponents (SyncVectorPatch.cpp:203-~630): along
 // dimension d the low neighbour's planes [n, n+gsp) and this patch's planes
 // [0, gsp), gsp = 1 + 2*os + dual_d, are replaced by (low + this).
-void sum_ghosts(Sim &S)
+// arr(patch, component) gives a patch's array of J component JX+c.
+template <class Arr>
+void sum_ghosts_on(const Sim &S, Arr arr)
 {
     const int npt = (int)S.patches.size();
     for (int d = 0; d < S.nd; ++d) {
@@ -259,12 +261,11 @@ void sum_ghosts(Sim &S)
             const int n = S.p.patch_size[d];
 #pragma omp parallel for schedule(static)
             for (int ip = 0; ip < npt; ++ip) {
-                OPatch &P = S.patches[ip];
+                const OPatch &P = S.patches[ip];
                 int lc[3] = {P.pc[0], P.pc[1], P.pc[2]};
                 lc[d] = wrapi(lc[d] - 1, S.npatch[d]);
-                OPatch &L = S.patches[S.pid(lc[0], lc[1], lc[2])];
-                double *pt = P.F[c].data();
-                double *lo = L.F[c].data();
+                double *pt = arr(ip, c);
+                double *lo = arr(S.pid(lc[0], lc[1], lc[2]), c);
                 int r0[3] = {0, 0, 0}, r1[3] = {dm[0], dm[1], dm[2]};
                 r1[d] = gsp;
                 for (int i = r0[0]; i < r1[0]; ++i)
@@ -282,12 +283,19 @@ void sum_ghosts(Sim &S)
     }
 }
 
+void sum_ghosts(Sim &S)
+{
+    sum_ghosts_on(S, [&](int ip, int c) { return S.patches[ip].F[c].data(); });
+}
+
 // --------------------------------------------------------- B exchange
 // SyncVectorPatch::exchangeB -> exchangeAllComponentsAlong{X,Y,Z}
 // (SyncVectorPatch.cpp:667, 1441-1830): for the two B components dual in d,
 // this patch's planes [0, os) <- low neighbour's [n, n+os), and the low
 // neighbour's [n+gsp, n+gsp+os) <- this patch's [gsp, gsp+os), gsp = os+2.
-void exchange_B(Sim &S)
+// arr(patch, component) gives a patch's array of B component BX+c.
+template <class Arr>
+void exchange_B_on(const Sim &S, Arr arr)
 {
     const int npt = (int)S.patches.size();
     for (int d = 0; d < S.nd; ++d) {
@@ -297,12 +305,11 @@ void exchange_B(Sim &S)
             const int os = S.os, gsp = os + 2, n = S.p.patch_size[d];
 #pragma omp parallel for schedule(static)
             for (int ip = 0; ip < npt; ++ip) {
-                OPatch &P = S.patches[ip];
+                const OPatch &P = S.patches[ip];
                 int lc[3] = {P.pc[0], P.pc[1], P.pc[2]};
                 lc[d] = wrapi(lc[d] - 1, S.npatch[d]);
-                OPatch &L = S.patches[S.pid(lc[0], lc[1], lc[2])];
-                double *pt = P.F[c].data();
-                double *lo = L.F[c].data();
+                double *pt = arr(ip, c);
+                double *lo = arr(S.pid(lc[0], lc[1], lc[2]), c);
                 int r1[3] = {dm[0], dm[1], dm[2]};
                 r1[d] = os;
                 for (int i = 0; i < r1[0]; ++i)
@@ -321,6 +328,11 @@ void exchange_B(Sim &S)
             }
         }
     }
+}
+
+void exchange_B(Sim &S)
+{
+    exchange_B_on(S, [&](int ip, int c) { return S.patches[ip].F[c].data(); });
 }
 
 void maxwell(Sim &S)
@@ -707,6 +719,39 @@ int ora_rho(void *h, double *g)
                         }
             }
         }
+    return 0;
+}
+
+// Offsets of the three components inside one patch's block of a
+// patch-major flat array (components first..first+2 of the patch layout).
+static void flat_offsets(const Sim &S, int first, size_t off[4])
+{
+    off[0] = 0;
+    for (int c = 0; c < 3; ++c) off[c + 1] = off[c] + S.fsize(first + c);
+}
+
+// J ghost sum on patch-major flat arrays: patch p's block at p * (|Jx| +
+// |Jy| + |Jz|), holding Jx, Jy, Jz in the per-patch ghosted layout.
+int ora_sum_j_ghosts(const ora_params *p, double *J)
+{
+    Sim *S = (Sim *)ora_create(p);
+    if (!S) return -1;
+    size_t off[4];
+    flat_offsets(*S, JX, off);
+    sum_ghosts_on(*S, [&](int ip, int c) { return J + (size_t)ip * off[3] + off[c - JX]; });
+    ora_destroy(S);
+    return 0;
+}
+
+// B exchange on patch-major flat arrays [patch][Bx|By|Bz] (ghosted layout).
+int ora_exchange_b(const ora_params *p, double *B)
+{
+    Sim *S = (Sim *)ora_create(p);
+    if (!S) return -1;
+    size_t off[4];
+    flat_offsets(*S, BX, off);
+    exchange_B_on(*S, [&](int ip, int c) { return B + (size_t)ip * off[3] + off[c - BX]; });
+    ora_destroy(S);
     return 0;
 }
 

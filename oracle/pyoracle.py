@@ -80,6 +80,8 @@ def lib():
         L.ora_yee3d_mf.argtypes = [_D] * 6 + [_I3, ctypes.c_double, _D3]
         L.ora_yee2d_ma.argtypes = [_D] * 9 + [_I2, ctypes.c_double, _D2]
         L.ora_yee2d_mf.argtypes = [_D] * 6 + [_I2, ctypes.c_double, _D2]
+        L.ora_sum_j_ghosts.argtypes = [ctypes.POINTER(OraParams), _D]
+        L.ora_exchange_b.argtypes = [ctypes.POINTER(OraParams), _D]
         _lib = L
     return _lib
 
@@ -277,3 +279,35 @@ def yee2d_ma(E, B, J, np2, dt, dx):
 
 def yee2d_mf(B, E, np2, dt, dx):
     lib().ora_yee2d_mf(*[dptr(a) for a in B], *[dptr(a) for a in E], _i(np2), float(dt), _d(dx))
+
+
+# ---- multi-patch synchronisation on patch-major flat arrays --------------
+def _params(ndim, order, n_cells, patch_size):
+    p = OraParams()
+    p.ndim, p.order = ndim, order
+    nc = list(n_cells) + [1] * (3 - len(n_cells))
+    ps = list(patch_size) + [1] * (3 - len(patch_size))
+    for d in range(3):
+        p.n_cells[d], p.patch_size[d], p.cell_length[d] = nc[d], ps[d], 1.0
+    p.timestep = 0.1
+    return p
+
+
+def patch_block_dims(ndim, order, patch_size, dual):
+    """Dims of the three components of one patch block (dual: 3x3 flags)."""
+    ps = list(patch_size) + [1] * (3 - len(patch_size))
+    return [tuple(ps[d] + 2 * order + 1 + dual[c][d] if d < ndim else 1 for d in range(3)) for c in range(3)]
+
+
+def sum_j_ghosts(ndim, order, n_cells, patch_size, J):
+    """J: flat patch-major float64 array [patch][Jx|Jy|Jz], updated in place."""
+    p = _params(ndim, order, n_cells, patch_size)
+    if lib().ora_sum_j_ghosts(ctypes.byref(p), dptr(J)) != 0:
+        raise ValueError("oracle rejected the geometry")
+
+
+def exchange_b(ndim, order, n_cells, patch_size, B):
+    """B: flat patch-major float64 array [patch][Bx|By|Bz], updated in place."""
+    p = _params(ndim, order, n_cells, patch_size)
+    if lib().ora_exchange_b(ctypes.byref(p), dptr(B)) != 0:
+        raise ValueError("oracle rejected the geometry")

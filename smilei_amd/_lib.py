@@ -89,6 +89,24 @@ SIGNATURES = {
                         + [_VP] * 5 + [_VP] * 2 + [ctypes.c_int, _VP]),
     "smi_maxwell_ampere_3d": (ctypes.c_int, [_VP] * 9 + [_I3, ctypes.c_double, _D3, _VP]),
     "smi_maxwell_faraday_3d": (ctypes.c_int, [_VP] * 6 + [_I3, ctypes.c_double, _D3, _VP]),
+    "smi_density_3d": (ctypes.c_int, [ctypes.c_int, _VP, _I3, _I3, _D3, ctypes.c_double, ctypes.c_int64]
+                       + [_VP] * 4 + [_VP]),
+    "smi_density_2d": (ctypes.c_int, [ctypes.c_int, _VP, ctypes.c_int * 2, ctypes.c_int * 2, ctypes.c_double * 2,
+                                      ctypes.c_double, ctypes.c_int64] + [_VP] * 3 + [_VP]),
+    "smi_maxwell_ampere_2d": (ctypes.c_int, [_VP] * 9 + [ctypes.c_int * 2, ctypes.c_double, ctypes.c_double * 2,
+                                                         _VP]),
+    "smi_maxwell_faraday_2d": (ctypes.c_int, [_VP] * 6 + [ctypes.c_int * 2, ctypes.c_double, ctypes.c_double * 2,
+                                                          _VP]),
+    "smi_current_deposition_3d": (ctypes.c_int, [_VP] * 3 + [ctypes.c_int] * 3 + [_VP] * 6
+                                  + [ctypes.c_uint] * 3 + [_VP] * 3 + [ctypes.c_uint] + [ctypes.c_double] * 7
+                                  + [ctypes.c_int] * 6 + [ctypes.c_bool]),
+    "smi_density_deposition_3d": (ctypes.c_int, [_VP, ctypes.c_int] + [_VP] * 6 + [ctypes.c_uint] * 3 + [_VP] * 3
+                                  + [ctypes.c_uint] + [ctypes.c_double] * 7 + [ctypes.c_int] * 6
+                                  + [ctypes.c_bool]),
+    "smi_sum_j_ghosts": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I3, _I3, _VP, _VP]),
+    "smi_exchange_b": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I3, _I3, _VP, _VP]),
+    "smi_sort_exchange": (ctypes.c_int, [ctypes.c_int, _I3, _I3, _D3, ctypes.c_int64] + [_VP] * 7 + [_VP] * 4
+                          + [_VP]),
 }
 
 _lib = None

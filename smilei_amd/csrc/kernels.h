@@ -87,6 +87,9 @@ void launch_det_deposit(const Geom &g, const DetSpecies *species, int nspecies, 
 // J ghost sum along each dimension (SyncVectorPatch::sumAllComponents) and
 // copy of the patch interiors into the global arrays.
 void launch_ghost_sum_and_fold(const Geom &g, double *pj, const JPtrs &J, hipStream_t st);
+// The x, y, z passes of the ghost sum alone, on the patch-major ghosted J
+// array pj ([patch][Jx|Jy|Jz], SyncVectorPatch::sumAllComponents).
+void launch_ghost_sum(const Geom &g, double *pj, hipStream_t st);
 int64_t patch_j_size(const Geom &g);  // doubles per patch (Jx+Jy+Jz)
 
 // ---- sort (sort.hip) --------------------------------------------------

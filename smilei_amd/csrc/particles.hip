@@ -714,7 +714,7 @@ void launch_det_deposit(const Geom &g, const DetSpecies *species, int nspecies, 
     SMI_LAUNCH_CHECK();
 }
 
-void launch_ghost_sum_and_fold(const Geom &g, double *pj, const JPtrs &J, hipStream_t st)
+void launch_ghost_sum(const Geom &g, double *pj, hipStream_t st)
 {
     const PJLayout L = pj_layout(g);
     for (int d = 0; d < g.nd; ++d) {
@@ -729,6 +729,11 @@ void launch_ghost_sum_and_fold(const Geom &g, double *pj, const JPtrs &J, hipStr
         hipLaunchKernelGGL(k_ghost_sum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, g, d, pj);
         SMI_LAUNCH_CHECK();
     }
+}
+
+void launch_ghost_sum_and_fold(const Geom &g, double *pj, const JPtrs &J, hipStream_t st)
+{
+    launch_ghost_sum(g, pj, st);
     hipLaunchKernelGGL(k_fold_interior, dim3((unsigned)((g.ncells + 255) / 256)), dim3(256), 0, st, g,
                        (const double *)pj, J);
     SMI_LAUNCH_CHECK();

@@ -76,3 +76,24 @@ def test_per_operator_entry_points_validate_arguments():
     assert rc == -1 and "order" in L.smi_last_error().decode()
     rc = L.smi_push_boris(3, -1.0, 0.0, 0.1, 0, *[None] * 6, None, None, None, None)
     assert rc == -1 and "mass" in L.smi_last_error().decode()
+
+
+def test_seam_and_sync_entry_points_validate_arguments():
+    """Geometry and bin checks fail before any device work."""
+    L = _lib.load()
+    I3 = ctypes.c_int * 3
+    D3 = ctypes.c_double * 3
+    # patch size not a multiple of 4
+    rc = L.smi_sum_j_ghosts(3, 2, I3(32, 32, 32), I3(16, 10, 16), ctypes.c_void_p(1), None)
+    assert rc == -1 and "multiple of 4" in L.smi_last_error().decode()
+    rc = L.smi_exchange_b(2, 4, I3(32, 32, 1), I3(16, 16, 1), ctypes.c_void_p(1), None)
+    assert rc == -1 and "2D order" in L.smi_last_error().decode()
+    rc = L.smi_sort_exchange(3, I3(32, 32, 32), I3(16, 16, 16), D3(0.1, 0.1, 0.1), 10, *[None] * 7, None, None,
+                             None, None, None)
+    assert rc == -1 and "null particle" in L.smi_last_error().decode()
+    args = [None] * 3 + [1] * 3 + [None] * 6 + [0, 1, 1] + [None] * 3 + [0] + [1.0] * 7 + [0] * 3 + [13, 13, 1, False]
+    rc = L.smi_current_deposition_3d(*args)
+    assert rc == -1 and "bin counts" in L.smi_last_error().decode()
+    args = [None, 1] + [None] * 6 + [2, 2, 2] + [None] * 3 + [0] + [1.0] * 7 + [0] * 3 + [13, 13, 2, False]
+    rc = L.smi_density_deposition_3d(*args)
+    assert rc == -1 and "not_spectral" in L.smi_last_error().decode()
