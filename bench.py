@@ -152,14 +152,42 @@ def run_gpu(args, W, form):
                 energies=e.tolist(), energies0=e0.tolist(), tuned=tuned)
 
 
+def energy_errors(ea, eb):
+    """Energy differences of run a against run b over the sampled steps
+    (rows: samples; columns: Uelm, Ukin of each species): the largest
+    |dUelm| / Uelm(b), |dUkin| / Ukin(b) (each energy against itself, at the
+    same step), and the same differences over the initial total energy."""
+    ua, ub = ea[:, 0], eb[:, 0]
+    ka, kb = ea[:, 1:].sum(1), eb[:, 1:].sum(1)
+    tot0 = abs(eb[0].sum())
+    live = np.abs(ub) > 0  # Uelm is zero at step 0 when the fields start at zero
+    return {
+        "field_vs_field": float((np.abs(ua - ub)[live] / np.abs(ub[live])).max()) if live.any() else 0.0,
+        "kinetic_vs_kinetic": float((np.abs(ka - kb) / np.abs(kb)).max()),
+        "field_vs_total": float(np.abs(ua - ub).max() / tot0),
+        "kinetic_vs_total": float(np.abs(ka - kb).max() / tot0),
+        "total_vs_total": float(np.abs(ea.sum(1) - eb.sum(1)).max() / tot0),
+    }
+
+
 def run_accuracy(args, W):
-    """Both forms of the workload from the same seeds for args.accuracy
-    steps (energies every 100 steps): the mixed form's energies against the
-    FP64 form's, relative to the initial total, and each form's Gauss-law
-    drift max|(div E - rho)(t) - (div E - rho)(0)| / max|rho(0)|."""
+    """Three runs of the workload from the same seeds for args.accuracy
+    steps, energies every 100 steps: the mixed form, the FP64 form, and the
+    FP64 form again with another launch shape of the particle kernel (128
+    threads per cluster, XCD-aware order: the same arithmetic summed in
+    another order).  The third run measures the floor: how far two FP64 runs
+    that differ only by the order of the atomic J sums drift apart.  Reports
+    each energy's error against itself and against the initial total, and
+    each run's Gauss-law drift max|(div E - rho)(t) - (div E - rho)(0)| /
+    max|rho(0)|."""
+    import smilei_amd as sa
+    runs = [("mixed", "mixed", None), ("fp64", "fp64", None), ("fp64_alt", "fp64", {sa.TUNE_THREADS: 128,
+                                                                                   sa.TUNE_XCD: 1})]
     hist = {}
-    for form in ("mixed", "fp64"):
+    for label, form, tuning in runs:
         sim, per_species = make_sim(args, W, form)
+        for k, v in (tuning or {}).items():
+            sim.set_tuning(k, v)
         g0 = sim.gauss()
         rho0 = float(np.abs(sim.rho()).max())
         e = [sim.energies().tolist()]
@@ -169,24 +197,24 @@ def run_accuracy(args, W):
             sim.step(k)
             done += k
             e.append(sim.energies().tolist())
-            print("accuracy %s: step %d, energies %s" % (form, done, e[-1]), file=sys.stderr, flush=True)
+            print("accuracy %s: step %d, energies %s" % (label, done, e[-1]), file=sys.stderr, flush=True)
         drift = float(np.abs(sim.gauss() - g0).max() / rho0)
-        hist[form] = dict(energies=np.array(e), gauss_drift=drift, npart=sum(per_species))
+        hist[label] = dict(energies=np.array(e), gauss_drift=drift, npart=sum(per_species))
         sim.close()
-    em, ef = hist["mixed"]["energies"], hist["fp64"]["energies"]
-    tot0 = abs(ef[0].sum())
+    em, ef, ea = hist["mixed"]["energies"], hist["fp64"]["energies"], hist["fp64_alt"]["energies"]
     out = {
         "accuracy_steps": args.accuracy,
         "workload": W["desc"],
         "order": args.order,
         "particles": hist["mixed"]["npart"],
-        "field_energy_error": float(np.abs(em[:, 0] - ef[:, 0]).max() / tot0),
-        "kinetic_energy_error": float(np.abs(em[:, 1:].sum(1) - ef[:, 1:].sum(1)).max() / tot0),
-        "total_energy_error": float(np.abs(em.sum(1) - ef.sum(1)).max() / tot0),
+        "mixed_vs_fp64": energy_errors(em, ef),
+        "fp64_floor": energy_errors(ea, ef),
         "gauss_drift_mixed": hist["mixed"]["gauss_drift"],
         "gauss_drift_fp64": hist["fp64"]["gauss_drift"],
+        "gauss_drift_fp64_alt": hist["fp64_alt"]["gauss_drift"],
         "energy_change_mixed": float(em[-1].sum() / em[0].sum() - 1),
         "energy_change_fp64": float(ef[-1].sum() / ef[0].sum() - 1),
+        "energies": {k: v["energies"].tolist() for k, v in hist.items()},
     }
     print(json.dumps(out))
 

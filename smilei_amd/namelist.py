@@ -15,7 +15,10 @@ Supported (the periodic, Yee, Boris, momentum-conserving hot path):
   (2, 4), two of cell_length / grid_length / number_of_cells,
   number_of_patches, timestep or timestep_over_CFL, simulation_time or
   number_of_timesteps (``n_time = int(simulation_time / timestep)``,
-  Params.cpp:1154), periodic EM_boundary_conditions, random_seed;
+  Params.cpp:1154), periodic EM_boundary_conditions, random_seed,
+  solve_poisson (default True: when the species' charge density is not
+  zero everywhere, E starts as its periodic electrostatic field, solved
+  exactly for the Yee grid's Laplacian; see ``poisson_e``);
 * ``Species``: mass, charge, particles_per_cell, number_density or
   charge_density (numbers or profiles of the position), temperature in
   m_e c^2 (one value or three equal values), mean_velocity along x (added
@@ -330,9 +333,58 @@ def _density(s):
     return (lambda x, y, z: np.abs(np.asarray(cd(x, y, z), dtype=float)) / q) if callable(cd) else abs(cd) / q
 
 
+def poisson_e(rho, dx):
+    """Electrostatic field of a periodic charge density on the Yee grid.
+
+    Solves -lap(phi) = rho - <rho> with the discrete 5/7-point Laplacian of
+    the primal grid (the operator the reference's conjugate-gradient solver
+    inverts, ElectroMagn3D::compute_Ap, ElectroMagn3D.cpp:312-330) exactly,
+    by its eigen-decomposition: the Laplacian is diagonal in the discrete
+    Fourier basis with eigenvalues -sum_d (2 sin(pi m_d / N_d) / dx_d)^2.
+    E_d = -(phi[i] - phi[i-1]) / dx_d sits at the dual index i ((i - 1/2) dx_d)
+    as in ElectroMagn3D::initE (ElectroMagn3D.cpp:529-), so the discrete
+    div E equals rho - <rho> to rounding.  The mean of rho (the k = 0 mode,
+    which no periodic field can carry) is left out, which is what the
+    reference's centring of E (VectorPatch::solvePoisson, VectorPatch.cpp:
+    1645-1670) amounts to.  rho has the simulation's dimensions; returns one
+    array per dimension."""
+    rho = np.asarray(rho, dtype=np.float64)
+    nd = rho.ndim
+    rhat = np.fft.fftn(rho)
+    k2 = np.zeros(rho.shape)
+    for d in range(nd):
+        m = np.arange(rho.shape[d])
+        kd = (2.0 * np.sin(np.pi * m / rho.shape[d]) / dx[d]) ** 2
+        shp = [1] * nd
+        shp[d] = rho.shape[d]
+        k2 = k2 + kd.reshape(shp)
+    k2.flat[0] = 1.0
+    phihat = rhat / k2
+    phihat.flat[0] = 0.0
+    phi = np.fft.ifftn(phihat).real
+    return [-(phi - np.roll(phi, 1, axis=d)) / dx[d] for d in range(nd)]
+
+
+def solve_poisson(sim):
+    """Main.solve_poisson at t = 0 (Smilei.cpp:329-332): when the charge
+    density of the species is not zero everywhere, E becomes the periodic
+    electrostatic field of that density (poisson_e).  Returns whether it
+    was solved."""
+    nd = sim.ndim
+    rho = sim.rho().reshape(sim.shape[:nd])
+    if not np.any(rho):
+        return False
+    E = poisson_e(rho, sim.dx[:nd])
+    for d, name in enumerate(("Ex", "Ey", "Ez")[:nd]):
+        sim.set_field(name, E[d].reshape(sim.shape))
+    return True
+
+
 def build(nl, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True):
     """Create the Simulation of a loaded namelist: species in namelist order,
-    then the external fields."""
+    the electrostatic field of their charge when Main.solve_poisson (the
+    reference default), then the external fields (added onto it, as
+    ElectroMagn::applyExternalFields adds its profiles)."""
     m = nl.main
     patch = [n // p for n, p in zip(m.number_of_cells, m.number_of_patches)]
     sim = Simulation(m.geometry, m.interpolation_order, m.number_of_cells, patch, m.cell_length, m.timestep,
@@ -364,6 +416,8 @@ def build(nl, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True):
             sid = sim.add_species(s.name, s.mass, s.charge, max(cap, 1))
             P.create_species(sim, sid, dens, ppc, theta=theta, drift_ux=drift, seed=seed + 1 + k)
         ids[s.name] = sid
+    if m.solve_poisson:
+        solve_poisson(sim)
     if nl.external_fields:
         P.apply_external_fields(sim, {e.field: e.profile for e in nl.external_fields})
     return sim

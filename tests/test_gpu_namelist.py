@@ -157,3 +157,54 @@ def test_signals_dump_and_stop_or_go_on(tmp_path):
     assert [sa.Simulation.restart(str(d)).steps_done for d in dumps] == [5, 9]
     # the handlers in place before the run are back
     assert (signal.getsignal(signal.SIGUSR1), signal.getsignal(signal.SIGUSR2)) == before
+
+
+NONNEUTRAL = """
+import math as m
+dx = 0.2
+Main(geometry = "3Dcartesian", interpolation_order = 2, timestep_over_CFL = 0.95,
+     number_of_timesteps = 20, cell_length = [dx]*3, number_of_cells = [16, 16, 16],
+     number_of_patches = [2, 2, 2], EM_boundary_conditions = [["periodic"]], random_seed = 9%s)
+Species(name = "proton", position_initialization = "random", momentum_initialization = "mj",
+        particles_per_cell = 4, mass = 1836.0, charge = 1.0, charge_density = 1.,
+        temperature = [0.01], boundary_conditions = [["periodic", "periodic"]] * 3)
+Species(name = "electron", position_initialization = "random", momentum_initialization = "mj",
+        particles_per_cell = 4, mass = 1.0, charge = -1.0, charge_density = 1.,
+        temperature = [0.01], boundary_conditions = [["periodic", "periodic"]] * 3)
+ExternalField(field = "Bz", profile = lambda x, y, z: 0.05 * m.cos(2 * m.pi * x / 3.2))
+"""
+
+
+def test_namelist_solves_poisson_for_a_nonneutral_start():
+    """Electrons and protons at independent random positions: rho != 0
+    locally, so (solve_poisson defaults to True, Smilei.cpp:329-332) E starts
+    as the electrostatic field of rho and div E - rho vanishes to rounding at
+    step 0 (the total charge is zero); with solve_poisson = False E starts at
+    zero and the residual is -rho."""
+    sim = N.build(N.load(NONNEUTRAL % ""))
+    rho = sim.rho()
+    assert np.abs(rho).max() > 0.1
+    assert np.abs(sim.gauss()).max() < 1e-12 * np.abs(rho).max()
+    off = N.build(N.load(NONNEUTRAL % ", solve_poisson = False"))
+    np.testing.assert_array_equal(off.gauss(), -off.rho())
+
+
+def test_namelist_run_matches_oracle_bit_exact():
+    """A namelist-built simulation (parameters derived from the namelist:
+    timestep from timestep_over_CFL, patches from number_of_patches, species
+    from profiles and temperatures, E from Poisson, an external B profile)
+    run in the deterministic FP64 form equals the oracle started from the
+    same particles and fields, bit for bit, after 20 steps."""
+    from oracle import pyoracle as O
+    from tests.test_gpu_configs import compare, copy_to_oracle
+    nl = N.load(NONNEUTRAL % "")
+    sim = N.build(nl, deposit=sa.DEPOSIT_DETERMINISTIC)
+    dx = (0.2, 0.2, 0.2)
+    dt = 0.95 / math.sqrt(3 / 0.2 ** 2)
+    assert abs(nl.main.timestep - dt) < 1e-15
+    o = O.OracleSim(3, 2, (16, 16, 16), (8, 8, 8), dx, dt)
+    copy_to_oracle(sim, 2, [1836.0, 1.0], [1.0, -1.0], o)
+    n = nl.main.number_of_timesteps
+    sim.step(n)
+    o.step(n)
+    compare(sim, o, 2, exact=True)

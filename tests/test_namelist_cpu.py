@@ -2,6 +2,7 @@
 refusal of what the hot path does not implement (no device calls)."""
 import math
 
+import numpy as np
 import pytest
 
 from smilei_amd import namelist as N
@@ -183,3 +184,34 @@ def test_diag_and_checkpoint_blocks_are_checked():
     nl = N.load(THERMAL + "Checkpoints(dump_step = 5, exit_after_dump = False)\n")
     assert nl.checkpoints.dump_step == 5 and nl.checkpoints.exit_after_dump is False
     assert nl.diag_scalar.every == 2
+
+
+@pytest.mark.parametrize("shape,dx", [((24, 16, 20), (0.3, 0.2, 0.25)), ((32, 12), (0.125, 0.2))])
+def test_poisson_e_satisfies_discrete_gauss(shape, dx):
+    """The divergence on the Yee grid, (E[i+1] - E[i]) / dx per axis at the
+    primal node, of the field returned for a random periodic density is the
+    density minus its mean, to rounding; and E has zero mean."""
+    rng = np.random.default_rng(5)
+    rho = rng.standard_normal(shape)
+    E = N.poisson_e(rho, dx)
+    div = sum((np.roll(E[d], -1, axis=d) - E[d]) / dx[d] for d in range(len(shape)))
+    np.testing.assert_allclose(div, rho - rho.mean(), atol=1e-12 * np.abs(rho).max())
+    for e in E:
+        assert abs(e.mean()) < 1e-14 * np.abs(e).max()
+
+
+def test_poisson_e_points_away_from_positive_charge():
+    """A single positive cosine mode along x: E_x at the dual points is
+    -d(phi)/dx with phi = rho / K^2, i.e. E_x(x) = sin(k (x - dx/2)) *
+    (2 sin(k dx / 2) / dx) / K^2 with K = 2 sin(k dx / 2) / dx: the
+    discrete counterpart of E = sin(k x) / k, pointing away from the
+    density peak at x = 0."""
+    N_, dx = 32, 0.25
+    x = np.arange(N_) * dx
+    k = 2 * np.pi / (N_ * dx)
+    rho = np.cos(k * x)[:, None] * np.ones((1, 8))
+    E = N.poisson_e(rho, (dx, 0.5))
+    K = 2 * np.sin(k * dx / 2) / dx
+    expect = np.sin(k * (x - dx / 2)) / K
+    np.testing.assert_allclose(E[0][:, 0], expect, atol=1e-13)
+    np.testing.assert_allclose(E[1], 0.0, atol=1e-13)
