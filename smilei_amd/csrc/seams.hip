@@ -22,19 +22,21 @@
 #include "../../include/smilei_amd.h"
 #include "kernels.h"
 #include "pic_device.h"
+#include "seams_internal.h"
 
 using namespace smi;
 using namespace smi::dev;
+using namespace smi::seams;
 
-namespace {
-
-inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
-
-int op_fail(int code, const std::string &m)
+int smi::seams::op_fail(int code, const std::string &m)
 {
     set_error(m);
     return code;
 }
+
+namespace {
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 #define SEAM_CHECK()                                                              \
     do {                                                                          \
@@ -465,10 +467,12 @@ __global__ void k_exchange_b(Geom g, PatchLayout L, int d, double *__restrict__ 
     lo[at(e3)] = me[at(f3)];
 }
 
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Geometry of the per-device entry points (the checks of smi_create).
 // ---------------------------------------------------------------------------
-int make_geom(int ndim, int order, const int n_cells[3], const int patch_size[3], const double *cell_length,
+int smi::seams::make_geom(int ndim, int order, const int n_cells[3], const int patch_size[3], const double *cell_length,
               Geom &g)
 {
     if (ndim != 2 && ndim != 3) return op_fail(SMI_ERR_INVALID, "ndim must be 2 or 3");
@@ -511,8 +515,6 @@ int make_geom(int ndim, int order, const int n_cells[3], const int patch_size[3]
     g.dt = 1.0;
     return SMI_OK;
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Particle exchange + cell sort of one species over all patches of a device.
@@ -571,20 +573,13 @@ __global__ void k_bins_out(Geom g, const int32_t *__restrict__ cell_off, int32_t
     if (patch_count && b == 0) patch_count[p] = cell_off[c0 + g.cpp] - cell_off[c0];
 }
 
-// Scratch of one sort/exchange call, carved out of one buffer of 4-byte
-// words (each part aligned to 256 bytes).
-struct SortScratch {
-    SortBufs sb;
-    uint32_t *keys;
-    double *stage;
-    int *err;
-};
-
 constexpr int64_t kAlignWords = 64;
 
 int64_t align_words(int64_t w) { return (w + kAlignWords - 1) / kAlignWords * kAlignWords; }
 
-int64_t sort_scratch_words(const Geom &g, int64_t n)
+}  // namespace
+
+int64_t smi::seams::sort_scratch_words(const Geom &g, int64_t n)
 {
     const int64_t nc = g.ncells, m = n > 0 ? n : 1;
     return align_words(nc) + align_words(nc + 1) + 2 * align_words(m) + align_words(m / 17 + 1) + align_words(2) +
@@ -592,7 +587,7 @@ int64_t sort_scratch_words(const Geom &g, int64_t n)
            align_words(m) + align_words(2 * m) + align_words(1);
 }
 
-SortScratch carve_sort_scratch(const Geom &g, int64_t n, int32_t *base)
+SortScratch smi::seams::carve_sort_scratch(const Geom &g, int64_t n, int32_t *base)
 {
     const int64_t nc = g.ncells, m = n > 0 ? n : 1;
     SortScratch s{};
@@ -619,15 +614,7 @@ SortScratch carve_sort_scratch(const Geom &g, int64_t n, int32_t *base)
     return s;
 }
 
-struct SortArrays {
-    double *x, *y, *z, *px, *py, *pz, *w;
-    int16_t *charge;
-    int32_t *key, *patch_count, *bin_index;
-};
-
-// The sort/exchange on scratch s, all on stream st; returns after the
-// stream has drained.
-int sort_exchange_impl(const Geom &g, int64_t n, const SortArrays &A, SortScratch &s, hipStream_t st)
+int smi::seams::sort_exchange_impl(const Geom &g, int64_t n, const SortArrays &A, SortScratch &s, hipStream_t st)
 {
     launch_fill_i32(s.sb.counts, g.ncells, 0, st);
     launch_fill_i32((int32_t *)s.sb.stay, (g.ncells + 3) / 4, 0, st);
@@ -677,7 +664,7 @@ int sort_exchange_impl(const Geom &g, int64_t n, const SortArrays &A, SortScratc
     return SMI_OK;
 }
 
-int sort_exchange_args(int ndim, const int n_cells[3], const int patch_size[3], const double cell_length[3],
+int smi::seams::sort_exchange_args(int ndim, const int n_cells[3], const int patch_size[3], const double cell_length[3],
                        int64_t n, const SortArrays &A, Geom &g)
 {
     if (!n_cells || !patch_size || !cell_length) return op_fail(SMI_ERR_INVALID, "null geometry");
@@ -688,8 +675,6 @@ int sort_exchange_args(int ndim, const int n_cells[3], const int patch_size[3], 
         return op_fail(SMI_ERR_INVALID, "null particle array");
     return SMI_OK;
 }
-
-}  // namespace
 
 extern "C" {
 
