@@ -52,6 +52,14 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
 TRAFFIC_PER_PUSH = {"beam": {"mixed": 127.1, "fp64": 186.6}}
+# VALU instructions per push of the mixed particle kernel on the beam-plasma
+# workload (SQ_INSTS_VALU per launch / particles, rocprofv3 --pmc:
+# profiles/k_particles_mixed_beam1B_sq.md), and the chip's VALU issue peak:
+# 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD.
+VALU_INSTR_PER_PUSH = {"beam": {"mixed": 1146.0 / 64.0}}
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
+# configured run length of each workload (steps)
+RUN_STEPS = {"beam": 300, "thermal": 200, "lwfa": 0}
 
 
 def cfl_dt(dx, factor=0.95):
@@ -125,31 +133,70 @@ def make_sim(args, W, form):
     return sim, per_species
 
 
+def profile(sim, steps):
+    """Device time per kernel class over `steps` eager steps (HIP events on
+    the library's stream), ms summed over the steps."""
+    acc = {"particles": 0.0, "sort": 0.0, "j_reduce": 0.0, "maxwell": 0.0}
+    for _ in range(steps):
+        ms = sim.profile_step()
+        for k in acc:
+            acc[k] += ms[k]
+    return acc
+
+
 def run_gpu(args, W, form):
+    """One form along the workload's trajectory: W warm-up steps, the
+    autotuner, the K timed graph-replayed steps (the headline: wall clock
+    between two device synchronisations), P eager steps with per-kernel-class
+    events; then, with --run-steps R > 0, the trajectory continues to step
+    R in blocks of 50 graph-replayed steps timed with device
+    synchronisations, and P more eager steps are profiled at its end (the
+    kernel times late in the run, once the beam has bunched)."""
     sim, per_species = make_sim(args, W, form)
     npart = sum(per_species)
     e0 = sim.energies()
     sim.step(args.warmup)
     sim.synchronize()
     tuned = sim.autotune(reps=2) if not args.no_autotune else None
-    # timed region: eager steps with HIP events around each kernel class
-    acc = {"particles": 0.0, "sort": 0.0, "j_reduce": 0.0, "maxwell": 0.0}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ms = sim.profile_step()
-        for k in acc:
-            acc[k] += ms[k]
+    done = args.warmup
+    # timed region: K graph-replayed steps (the production path)
     sim.synchronize()
-    t1 = time.perf_counter()
-    # graph-replayed steps (the production path) for comparison
-    tg0 = time.perf_counter()
+    t0 = time.perf_counter()
     sim.step(args.steps)
     sim.synchronize()
-    tg1 = time.perf_counter()
-    e = sim.energies()
+    t1 = time.perf_counter()
+    done += args.steps
+    P = max(1, min(args.steps, 10))
+    acc = profile(sim, P)
+    done += P
+    out = dict(npart=npart, per_species=per_species, wall=t1 - t0, kernel_ms=acc, profiled_steps=P, tuned=tuned)
+    if args.run_steps > done:
+        blocks = []
+        while done < args.run_steps:
+            k = min(50, args.run_steps - done)
+            sim.synchronize()
+            b0 = time.perf_counter()
+            sim.step(k)
+            sim.synchronize()
+            blocks.append((done, k, time.perf_counter() - b0))
+            done += k
+        print("%s: trajectory to step %d done" % (form, done), file=sys.stderr, flush=True)
+        out["kernel_ms_late"] = profile(sim, P)
+        tot = sum(b[2] for b in blocks) + (t1 - t0)
+        nst = sum(b[1] for b in blocks) + args.steps
+        last, nlast = 0.0, 0
+        for first, k, t in reversed(blocks):
+            if nlast >= 50:
+                break
+            last += t
+            nlast += k
+        out["run"] = dict(first_step=args.warmup, last_step=done, steps=nst, ms_per_step=1e3 * tot / nst,
+                          ms_per_step_last=1e3 * last / nlast, last_steps=nlast,
+                          pushes_per_s=npart * nst / tot, pushes_per_s_last=npart * nlast / last)
+    out["energies"] = sim.energies().tolist()
+    out["energies0"] = e0.tolist()
     sim.close()
-    return dict(npart=npart, per_species=per_species, wall=t1 - t0, wall_graph=tg1 - tg0, kernel_ms=acc,
-                energies=e.tolist(), energies0=e0.tolist(), tuned=tuned)
+    return out
 
 
 def energy_errors(ea, eb):
@@ -289,6 +336,9 @@ def main():
     ap.add_argument("--accuracy", type=int, default=0,
                     help="instead of timing: run both forms N steps from the same seeds and print the mixed "
                          "form's energy errors and both forms' Gauss-law drift")
+    ap.add_argument("--run-steps", type=int, default=-1,
+                    help="follow the trajectory to this step after the timed steps and report its mean step time "
+                         "and that of its last 50 steps (-1: the workload's configured length, 0: off)")
     ap.add_argument("--cell-order", default="any", choices=["any", "canonical"],
                     help="order of particles inside a cell after the sort: 'any' (left to the atomic scatter, "
                          "as the reference's GPU sort) or the canonical order")
@@ -301,6 +351,8 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     W = workload(args.workload, args.ppc)
+    if args.run_steps < 0:
+        args.run_steps = RUN_STEPS[W["name"]]
     if args.accuracy > 0:
         run_accuracy(args, W)
         return
@@ -310,21 +362,27 @@ def main():
     for form in forms:
         g = run_gpu(args, W, form)
         npart, K = g["npart"], args.steps
-        t_part = g["kernel_ms"]["particles"] / 1e3 / K  # s per step, all species
+        t_part = g["kernel_ms"]["particles"] / 1e3 / g["profiled_steps"]  # s per step, all species
         achieved = BYTES_PER_PUSH[form] * npart / t_part / 1e9
         e0, e1 = np.array(g["energies0"]), np.array(g["energies"])
+        P = g["profiled_steps"]
         res[form] = dict(value=npart * K / g["wall"], ms_per_step=1e3 * g["wall"] / K,
-                         ms_per_step_graph=1e3 * g["wall_graph"] / K,
-                         kernel_ms_per_step={k: v / K for k, v in g["kernel_ms"].items()},
+                         kernel_ms_per_step={k: v / P for k, v in g["kernel_ms"].items()},
                          achieved=achieved, npart=npart, particles_per_species=g["per_species"],
                          autotuned=g["tuned"],
                          energy_change=float(e1.sum() / e0.sum() - 1), energies=g["energies"])
+        if "run" in g:
+            res[form]["run"] = g["run"]
+            res[form]["kernel_ms_per_step_late"] = {k: v / P for k, v in g["kernel_ms_late"].items()}
     head = forms[0]
     h = res[head]
     # HBM bytes per particle-kernel launch (one launch per step runs every
     # species of the workload)
     tpp = TRAFFIC_PER_PUSH.get(W["name"], {}).get(head)
     traffic = tpp * h["npart"] if tpp else None
+    # pushes per second of the particle kernel alone (its device time)
+    kernel_rate = h["npart"] / (h["kernel_ms_per_step"]["particles"] / 1e3)
+    vpp = VALU_INSTR_PER_PUSH.get(W["name"], {}).get(head)
     out = {
         "metric": "particle-pushes/s (gather+Boris+Esirkepov) 1xMI355X, mixed vs FP64; %HBM peak",
         "value": h["value"],
@@ -353,9 +411,14 @@ def main():
             "frac": h["achieved"] / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_bytes_per_push": tpp,
+            "achieved_traffic_GBs": tpp * kernel_rate / 1e9 if tpp else None,
             "measured_copy_GBs": copy_gbs,
             "kernel": "%s particle kernel (fused gather+Boris+key+Esirkepov), %d B/push algorithmic"
                       % (head, BYTES_PER_PUSH[head]),
+            "frac_note": "frac counts algorithmic particle bytes only; the kernel is bound by VALU issue and LDS "
+                         "(atomics, bank conflicts), not by HBM",
+            "valu_instr_per_push": vpp,
+            "valu_frac": vpp * kernel_rate / VALU_PEAK_WAVE_INSTR if vpp else None,
         },
     }
     if "mixed" in res and "fp64" in res:

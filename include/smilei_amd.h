@@ -159,6 +159,16 @@ int smi_get_field(smi_context *ctx, int field, double *global);
 /* Run n complete time steps (stream-ordered; returns after enqueueing). */
 int smi_step(smi_context *ctx, int nsteps);
 int smi_synchronize(smi_context *ctx);
+/* Species::time_frozen_ (src/Species/Species.cpp:568, 832-866): the
+ * species is not pushed, deposits no current and is not re-sorted while
+ * the step's time_dual = (steps done + 1.5) * timestep (Smilei.cpp:172,
+ * 487-488) is <= time_frozen; it still counts in smi_rho / smi_gauss and
+ * the energies.  0 (the default) never freezes. */
+int smi_set_time_frozen(smi_context *ctx, int species, double time_frozen);
+/* Steps done so far (smi_step advances it); a restart sets it so that
+ * time-dependent choices (frozen species) continue where the dump left. */
+int smi_set_step_count(smi_context *ctx, int64_t steps_done);
+int64_t smi_step_count(const smi_context *ctx);
 /* out[0] = Uelm, out[1+s] = Ukin of species s (DiagnosticScalar formulas). */
 int smi_energies(smi_context *ctx, double *out);
 /* The per-patch sums of DiagnosticScalar::compute

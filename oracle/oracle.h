@@ -36,6 +36,9 @@ void   *ora_create(const ora_params *p);
 void    ora_destroy(void *h);
 void    ora_set_threads(int n);
 int     ora_add_species(void *h, double mass, double charge);
+/* Species::time_frozen_: no push, deposit or exchange while the step's
+ * time_dual = (steps done + 1.5) * timestep <= time_frozen. */
+int     ora_set_time_frozen(void *h, int sp, double time_frozen);
 int     ora_set_particles(void *h, int sp, int64_t n,
                           const double *x, const double *y, const double *z,
                           const double *px, const double *py, const double *pz,

@@ -69,6 +69,7 @@ struct OPatch {
 
 struct SpeciesDef {
     double mass, charge;
+    double time_frozen = 0.;  // Species::time_frozen_
 };
 
 struct Sim {
@@ -80,7 +81,13 @@ struct Sim {
     int dims[NF][3];
     std::vector<OPatch> patches;
     std::vector<SpeciesDef> species;
+    int64_t steps = 0;  // time steps done
     double timers[4] = {0, 0, 0, 0};
+
+    // Species::dynamics pushes a species when time_dual > time_frozen
+    // (Species.cpp:568, 832-866), time_dual = (steps + 1.5) dt during the
+    // next step (Smilei.cpp:172, 487-488).
+    bool moves(size_t s) const { return ((double)steps + 1.5) * p.timestep > species[s].time_frozen; }
 
     int pid(int a, int b, int c) const { return (a * npatch[1] + b) * npatch[2] + c; }
     size_t fsize(int c) const { return (size_t)dims[c][0] * dims[c][1] * dims[c][2]; }
@@ -161,7 +168,7 @@ void patch_dynamics(Sim &S, OPatch &P)
     for (size_t s = 0; s < S.species.size(); ++s) {
         Parts &pt = P.sp[s];
         const int64_t n = (int64_t)pt.size();
-        if (n == 0) continue;
+        if (n == 0 || !S.moves(s)) continue;
         Ep.resize(3 * n); Bp.resize(3 * n); invgf.resize(n);
         dold.resize(3 * n); iold.resize(3 * n);
         if (S.nd == 3) {
@@ -388,7 +395,8 @@ void step(Sim &S)
 #pragma omp parallel for schedule(dynamic)
     for (int ip = 0; ip < npt; ++ip) patch_dynamics(S, S.patches[ip]);
     double t1 = now();
-    for (size_t s = 0; s < S.species.size(); ++s) exchange_and_sort(S, (int)s);
+    for (size_t s = 0; s < S.species.size(); ++s)
+        if (S.moves(s)) exchange_and_sort(S, (int)s);
     double t2 = now();
     sum_ghosts(S);
     double t3 = now();
@@ -398,6 +406,7 @@ void step(Sim &S)
     S.timers[1] += t2 - t1;
     S.timers[2] += t3 - t2;
     S.timers[3] += t4 - t3;
+    S.steps += 1;
 }
 
 // Global index of local point l of patch P along d (primal and dual share
@@ -455,6 +464,14 @@ void ora_destroy(void *h) { delete (Sim *)h; }
 void ora_set_threads(int n)
 {
     if (n > 0) omp_set_num_threads(n);
+}
+
+int ora_set_time_frozen(void *h, int s, double time_frozen)
+{
+    Sim *S = (Sim *)h;
+    if (s < 0 || s >= (int)S->species.size()) return -1;
+    S->species[s].time_frozen = time_frozen;
+    return 0;
 }
 
 int ora_add_species(void *h, double mass, double charge)

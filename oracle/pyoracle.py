@@ -48,6 +48,7 @@ def lib():
         L.ora_destroy.argtypes = [ctypes.c_void_p]
         L.ora_set_threads.argtypes = [ctypes.c_int]
         L.ora_add_species.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.ora_set_time_frozen.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.ora_set_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64] + [_D] * 7
         L.ora_num_particles.restype = ctypes.c_int64
         L.ora_num_particles.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -123,6 +124,10 @@ class OracleSim:
     def add_species(self, mass, charge):
         self.nspecies += 1
         return lib().ora_add_species(self.h, float(mass), float(charge))
+
+    def set_time_frozen(self, s, time_frozen):
+        if lib().ora_set_time_frozen(self.h, s, float(time_frozen)) != 0:
+            raise ValueError("bad species id")
 
     def set_particles(self, s, x, y, z, px, py, pz, w):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, px, py, pz, w)]

@@ -50,7 +50,6 @@ class Simulation:
                           precision=int(precision), deposit=int(deposit), use_graph=bool(use_graph),
                           interpolator=interpolator, pusher=pusher, maxwell_solver=maxwell_solver)
         self._species_args = []
-        self.steps_done = 0
         self.extra = {}
         nd = 2 if geometry == "2Dcartesian" else 3
         p = SmiParams()
@@ -90,6 +89,22 @@ class Simulation:
         self.species.append(name)
         self._species_args.append(dict(name=name, mass=float(mass), charge=float(charge), capacity=int(capacity)))
         return sid.value
+
+    def set_time_frozen(self, s, time_frozen):
+        """Species::time_frozen_: the species is not pushed, deposits no
+        current and is not re-sorted while a step's time_dual (steps done +
+        1.5, times the timestep) is at most time_frozen."""
+        check(self._L.smi_set_time_frozen(self.h, s, float(time_frozen)))
+        self._species_args[s]["time_frozen"] = float(time_frozen)
+
+    @property
+    def steps_done(self):
+        """Time steps done (the engine's counter)."""
+        return int(self._L.smi_step_count(self.h))
+
+    @steps_done.setter
+    def steps_done(self, n):
+        check(self._L.smi_set_step_count(self.h, int(n)))
 
     def set_particles(self, s, x, y, z, px, py, pz, w):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, px, py, pz, w)]
@@ -173,7 +188,6 @@ class Simulation:
     # ---- time loop
     def step(self, n=1):
         check(self._L.smi_step(self.h, int(n)))
-        self.steps_done += int(n)
 
     # ---- checkpoint / restart (the reference dumps HDF5, Checkpoint.cpp:212-587;
     # here a NumPy archive of the same state: every field array, every
@@ -205,6 +219,8 @@ class Simulation:
             for s, sp in enumerate(meta["species"]):
                 sid = sim.add_species(sp["name"], sp["mass"], sp["charge"], sp["capacity"])
                 sim.set_particles(sid, *[z["sp%d_%s" % (s, k)] for k in ("x", "y", "z", "px", "py", "pz", "w")])
+                if sp.get("time_frozen", 0.0):
+                    sim.set_time_frozen(sid, sp["time_frozen"])
             for f in FIELDS:
                 sim.set_field(f, z["field_" + f])
             sim.steps_done = int(meta["steps_done"])

@@ -64,6 +64,9 @@ SIGNATURES = {
     "smi_get_field": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
     "smi_step": (ctypes.c_int, [_VP, ctypes.c_int]),
     "smi_synchronize": (ctypes.c_int, [_VP]),
+    "smi_set_time_frozen": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double]),
+    "smi_set_step_count": (ctypes.c_int, [_VP, ctypes.c_int64]),
+    "smi_step_count": (ctypes.c_int64, [_VP]),
     "smi_energies": (ctypes.c_int, [_VP, _D]),
     "smi_scalars": (ctypes.c_int, [_VP, _D]),
     "smi_rho": (ctypes.c_int, [_VP, _D]),

@@ -42,6 +42,8 @@ struct SpeciesState {
     SpeciesK k{};
     SpeciesKM km{};
     double w_max = 0.;             // largest |weight| (fixed-point range of the mixed form)
+    double time_frozen = 0.;       // Species::time_frozen_: immobile while time_dual <= time_frozen
+    bool mirrored = false;         // both ping-pong buffers hold the current particles (frozen species)
 
     PartPtrs view(int b) const
     {
@@ -91,6 +93,8 @@ struct smi_context {
     double *rho = nullptr;         // diagnostics scratch (ncells)
     int *err = nullptr;            // device flag raised by kernels on corrupt particles
     int cur = 0;                   // particle buffer holding the current state
+    int64_t steps = 0;             // time steps done (time_dual of the next step = (steps + 1.5) dt)
+    std::vector<int> moving;       // species pushed by the next step, in id order (launch descriptors)
     bool mixed = false;            // SMI_MIXED form
     int interleave = 1;            // lane-interleaved cluster layout of the sort (tuning)
     int threads = 256;             // workgroup size of the particle kernel (tuning)
@@ -151,6 +155,7 @@ bool want_placed(const smi_context *c)
 void sort_species(smi_context *c, SpeciesState &S, const uint32_t *keys, bool counted, bool ordered,
                   const int32_t *tie = nullptr)
 {
+    if (!counted) S.mirrored = false;  // freshly loaded particles: one buffer holds them
     S.lay = want_interleaved(c);
     launch_sort(c->g, keys, S.n, S.sb, c->st, counted, ordered, S.lay, tie, counted && want_placed(c));
 }
@@ -168,10 +173,11 @@ void drop_graphs(smi_context *c)
 // (set k: input buffer k, output buffer 1-k), refreshed when species change.
 void upload_det(smi_context *c)
 {
-    if (c->sp.empty()) return;
+    if (c->moving.empty()) return;
     std::vector<DetSpecies> v;
     for (int k = 0; k < 2; ++k)
-        for (auto &s : c->sp) {
+        for (int id : c->moving) {
+            const SpeciesState &s = c->sp[id];
             DetSpecies d;
             d.sp = s.k;
             d.cell_off = s.sb.cell_off;
@@ -190,10 +196,11 @@ void upload_det(smi_context *c)
 // buffer k, output 1-k), refreshed when species change.
 void upload_mix(smi_context *c)
 {
-    if (c->sp.empty() || !c->mixed) return;
+    if (c->moving.empty() || !c->mixed) return;
     std::vector<MixSpecies> v;
     for (int k = 0; k < 2; ++k)
-        for (auto &s : c->sp) {
+        for (int id : c->moving) {
+            const SpeciesState &s = c->sp[id];
             MixSpecies m;
             m.k = s.km;
             m.A = s.mview(k);
@@ -231,17 +238,17 @@ void enqueue_particles(smi_context *c, int a)
     GatherFields GF{f.E[0], f.E[1], f.E[2], f.Bm[0], f.Bm[1], f.Bm[2]};
     if (c->mixed) {
         // species share the cluster's field and J tiles, kMaxMerged per launch
-        const int ns = (int)c->sp.size();
+        const int ns = (int)c->moving.size();
         for (int g0 = 0; g0 < ns; g0 += kMaxMerged) {
             const int n = std::min(kMaxMerged, ns - g0);
             double cr_max = 0.;
-            for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[q].km.cr_max);
+            for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[c->moving[q]].km.cr_max);
             launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->threads, c->xcd != 0, c->tile,
                                    c->persistent ? c->persistent_blocks : 0, c->d_work,
                                    c->d_work + g.npatches * g.clpp, c->st);
         }
     } else {
-        const int ns = (int)c->sp.size();
+        const int ns = (int)c->moving.size();
         for (int g0 = 0; g0 < ns; g0 += kMaxMerged)
             launch_particles(g, c->d_fp + a * ns + g0, std::min(kMaxMerged, ns - g0), GF, J, !det, c->err,
                              c->threads, c->xcd != 0, c->persistent ? c->persistent_blocks : 0, c->d_work,
@@ -252,10 +259,11 @@ void enqueue_particles(smi_context *c, int a)
 // FP64-form launch descriptors for both ping-pong states.
 void upload_fp(smi_context *c)
 {
-    if (c->sp.empty() || c->mixed) return;
+    if (c->moving.empty() || c->mixed) return;
     std::vector<FpSpecies> v;
     for (int k = 0; k < 2; ++k)
-        for (auto &s : c->sp) {
+        for (int id : c->moving) {
+            const SpeciesState &s = c->sp[id];
             FpSpecies f;
             f.k = s.k;
             f.A = s.view(k);
@@ -304,19 +312,64 @@ void enqueue_step(smi_context *c, bool timed)
     enqueue_particles(c, a);
     if (timed) SMI_HIP(hipEventRecord(c->ev[1], c->st));
     if (det && !c->sp.empty()) {
-        launch_det_deposit(g, c->d_det + a * c->d_det_n, (int)c->sp.size(), c->pj, c->err, c->st);
+        // every species' deposit (none: an empty J, still ghost-summed)
+        launch_det_deposit(g, c->d_det + a * (int)c->moving.size(), (int)c->moving.size(), c->pj, c->err, c->st);
         launch_ghost_sum_and_fold(g, c->pj, J, c->st);
     }
     if (timed) SMI_HIP(hipEventRecord(c->ev[2], c->st));
     launch_maxwell(g, f, c->st);
     if (timed) SMI_HIP(hipEventRecord(c->ev[3], c->st));
     const bool ordered = det || c->cell_order;
-    for (auto &s : c->sp)
+    for (int id : c->moving) {
+        SpeciesState &s = c->sp[id];
         sort_species(c, s, s.keys[1 - a], /*counted=*/true, ordered, want_rank(c) ? s.sb.rank : nullptr);
+    }
     if (timed) SMI_HIP(hipEventRecord(c->ev[4], c->st));
 }
 
 void flip(smi_context *c) { c->cur ^= 1; }
+
+// Species pushed by the step taken after c->steps steps: those with
+// time_dual = (steps + 1.5) dt > time_frozen (Species::dynamics,
+// Species.cpp:568 and 832-866; time_dual as advanced in Smilei.cpp:487-488).
+std::vector<int> moving_now(const smi_context *c)
+{
+    std::vector<int> m;
+    const double time_dual = ((double)c->steps + 1.5) * c->g.dt;
+    for (int id = 0; id < (int)c->sp.size(); ++id)
+        if (time_dual > c->sp[id].time_frozen) m.push_back(id);
+    return m;
+}
+
+// Before a step (outside any graph capture): a frozen species is left out
+// of the particle launch and the sort, so both ping-pong buffers must hold
+// its particles (the buffer index flips every step); the launch descriptors
+// follow the set of moving species, and graphs captured for another set are
+// dropped.
+void prepare_step(smi_context *c, bool force = false)
+{
+    std::vector<int> mv = moving_now(c);
+    for (int id = 0; id < (int)c->sp.size(); ++id) {
+        SpeciesState &S = c->sp[id];
+        if (S.mirrored || std::find(mv.begin(), mv.end(), id) != mv.end()) continue;
+        const size_t esz = c->mixed ? sizeof(float) : sizeof(double);
+        const int a = c->cur;
+        if (S.n > 0) {
+            for (int q = 0; q < 7; ++q)
+                SMI_HIP(hipMemcpyAsync(S.buf[1 - a][q], S.buf[a][q], S.n * esz, hipMemcpyDeviceToDevice, c->st));
+            SMI_HIP(hipMemcpyAsync(S.keys[1 - a], S.keys[a], S.n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                   c->st));
+        }
+        S.mirrored = true;
+    }
+    if (force || mv != c->moving) {
+        c->moving = mv;
+        drop_graphs(c);
+        upload_det(c);
+        upload_fp(c);
+        upload_mix(c);
+    }
+}
 
 // Largest current contribution |q| w_max / V * max(dx/dt) of a species: sets
 // the fixed-point unit of the mixed form's J tile.
@@ -532,6 +585,7 @@ int smi_add_species(smi_context *c, const char *name, double mass, double charge
     for (int d = 0; d < 3; ++d) s.km.dxdt[d] = (float)(c->g.dx[d] / c->g.dt);
     set_quantum(c->g, s, 1.0);
     c->sp.push_back(s);
+    c->moving = moving_now(c);
     // deterministic-deposit descriptors for both ping-pong parities
     (void)hipFree(c->d_det);
     c->d_det = nullptr;
@@ -1042,6 +1096,7 @@ int smi_step(smi_context *c, int nsteps)
     if (nsteps < 0) throw Error(SMI_ERR_INVALID, "nsteps < 0");
     SMI_HIP(hipSetDevice(c->p.device));
     for (int it = 0; it < nsteps; ++it) {
+        prepare_step(c);
         if (c->p.use_graph) {
             // graph[k]: one step reading particle buffer k (captured once)
             const int k = c->cur;
@@ -1066,10 +1121,34 @@ int smi_step(smi_context *c, int nsteps)
             enqueue_step(c, false);
         }
         flip(c);
+        ++c->steps;
     }
     return SMI_OK;
     SMI_CATCH
 }
+
+int smi_set_time_frozen(smi_context *c, int s, double time_frozen)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (s < 0 || s >= (int)c->sp.size()) throw Error(SMI_ERR_INVALID, "bad species id");
+    if (!(time_frozen == time_frozen)) throw Error(SMI_ERR_INVALID, "time_frozen is NaN");
+    c->sp[s].time_frozen = time_frozen;
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int smi_set_step_count(smi_context *c, int64_t steps_done)
+{
+    SMI_TRY
+    check_ctx(c);
+    if (steps_done < 0) throw Error(SMI_ERR_INVALID, "steps_done < 0");
+    c->steps = steps_done;
+    return SMI_OK;
+    SMI_CATCH
+}
+
+int64_t smi_step_count(const smi_context *c) { return c ? c->steps : -1; }
 
 int smi_synchronize(smi_context *c)
 {
@@ -1090,8 +1169,10 @@ int smi_profile_step(smi_context *c, float *ms)
     SMI_TRY
     check_ctx(c);
     SMI_HIP(hipSetDevice(c->p.device));
+    prepare_step(c);
     enqueue_step(c, true);
     flip(c);
+    ++c->steps;
     SMI_HIP(hipEventSynchronize(c->ev[4]));
     for (int k = 0; k < 4; ++k) SMI_HIP(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
     // ms[0] particles, ms[1] deterministic J reduction, ms[2] Maxwell, ms[3] sort
@@ -1161,6 +1242,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     check_ctx(c);
     if (reps < 1) throw Error(SMI_ERR_INVALID, "reps must be >= 1");
     SMI_HIP(hipSetDevice(c->p.device));
+    prepare_step(c);
     SMI_HIP(hipStreamSynchronize(c->st));
     drop_graphs(c);
     // candidates: cluster layout (tile-deposit forms) x J-tile shape (mixed)

@@ -26,7 +26,7 @@ Supported (the periodic, Yee, Boris, momentum-conserving hot path):
   (isotropic Maxwell-Juttner of temperature T/m, drawn exactly by the
   library) or "cold", position_initialization "random" or the name of an
   earlier species (copy of its positions), periodic boundary_conditions,
-  pusher "boris";
+  pusher "boris", time_frozen (immobile until then: no push, no current);
 * ``ExternalField`` (initial fields, applied as applyExternalFields does);
 * ``DiagScalar``: ``scalars.txt`` in the reference's text layout (every,
   vars, precision; the energy balance, per-species and per-field scalars;
@@ -39,7 +39,7 @@ Supported (the periodic, Yee, Boris, momentum-conserving hot path):
 Blocks that change nothing on this path (load balancing, vectorisation and
 the other diagnostics) are accepted and recorded; blocks
 that would change the physics (lasers, moving window, collisions,
-injectors, filters, antennas, walls, radiation) raise ``NamelistError``
+injectors, filters, antennas, walls, radiation, ionisation) raise ``NamelistError``
 rather than run a different simulation.
 """
 import glob
@@ -201,8 +201,8 @@ class Namelist:
                     raise NamelistError("Species %s: only periodic boundary conditions are supported" % s.name)
             if s.pusher != "boris":
                 raise NamelistError("Species %s: pusher %r is not supported (boris only)" % (s.name, s.pusher))
-            if s.time_frozen and s.time_frozen > 0:
-                raise NamelistError("Species %s: frozen species are not supported" % s.name)
+            if callable(s.time_frozen) or not float(s.time_frozen) >= 0:
+                raise NamelistError("Species %s: time_frozen must be a number >= 0" % s.name)
             if s.ionization_model != "none" or s.radiation_model != "none":
                 raise NamelistError("Species %s: ionisation / radiation are not supported" % s.name)
             mv = list(s.mean_velocity) + [0.0] * (3 - len(s.mean_velocity))
@@ -416,6 +416,8 @@ def build(nl, precision=PLAIN_FP64, deposit=DEPOSIT_ATOMIC, use_graph=True):
             sid = sim.add_species(s.name, s.mass, s.charge, max(cap, 1))
             P.create_species(sim, sid, dens, ppc, theta=theta, drift_ux=drift, seed=seed + 1 + k)
         ids[s.name] = sid
+        if float(s.time_frozen) > 0:
+            sim.set_time_frozen(sid, float(s.time_frozen))
     if m.solve_poisson:
         solve_poisson(sim)
     if nl.external_fields:

@@ -109,7 +109,8 @@ static int mode_symbols(void)
         (fn_t)smi_init_species_profile, (fn_t)smi_init_species_profile_mj,
         (fn_t)smi_num_particles, (fn_t)smi_get_particles, (fn_t)smi_patch_counts,
         (fn_t)smi_set_field, (fn_t)smi_get_field, (fn_t)smi_step,
-        (fn_t)smi_synchronize, (fn_t)smi_energies, (fn_t)smi_scalars, (fn_t)smi_rho,
+        (fn_t)smi_synchronize, (fn_t)smi_set_time_frozen, (fn_t)smi_set_step_count,
+        (fn_t)smi_step_count, (fn_t)smi_energies, (fn_t)smi_scalars, (fn_t)smi_rho,
         (fn_t)smi_gauss, (fn_t)smi_profile_step, (fn_t)smi_stream,
         (fn_t)smi_set_tuning, (fn_t)smi_get_tuning, (fn_t)smi_autotune,
         (fn_t)smi_interpolate_3d, (fn_t)smi_interpolate_2d, (fn_t)smi_push_boris,

@@ -642,3 +642,32 @@ def test_thermal_energy_history_short():
         hist.append(S.energies().sum())
     hist = np.array(hist)
     assert np.abs(hist / hist.mean() - 1).max() < 1e-2
+
+
+def test_frozen_species_stays_put_until_time_frozen():
+    """Species::time_frozen_: a species is pushed by the step whose
+    time_dual = (steps done + 1.5) dt exceeds time_frozen.  With
+    time_frozen = 4.2 dt the protons stay bit-identical through steps 1-3
+    (time_dual 1.5, 2.5, 3.5 dt), the electrons move and the protons' charge
+    still counts in rho; from step 4 (4.5 dt) the protons move."""
+    shape, patch, dx = (16, 16, 16), (8, 8, 8), (0.2, 0.2, 0.2)
+    dt = cfl_dt(dx)
+    S = O.OracleSim(3, 2, shape, patch, dx, dt)
+    ps = thermal(shape, dx, 2, 0.3, 12)
+    pe = thermal(shape, dx, 2, 0.3, 13)
+    e = S.add_species(1.0, -1.0)
+    p = S.add_species(1.0, 1.0)  # light ions: they would move at once
+    S.set_particles(e, pe["x"], pe["y"], pe["z"], pe["px"], pe["py"], pe["pz"], pe["w"])
+    S.set_particles(p, ps["x"], ps["y"], ps["z"], ps["px"], ps["py"], ps["pz"], ps["w"])
+    S.set_time_frozen(p, 4.2 * dt)
+    p0, e0 = S.get_particles(p), S.get_particles(e)
+    g0 = S.gauss()
+    S.step(3)
+    p3, e3 = S.get_particles(p), S.get_particles(e)
+    for k in ("x", "y", "z", "px", "py", "pz", "w", "key", "patch"):
+        np.testing.assert_array_equal(p3[k], p0[k], err_msg=k)
+    assert not np.array_equal(e3["x"], e0["x"])
+    # Gauss stays put with the frozen charge included in rho
+    assert np.abs(S.gauss() - g0).max() < 1e-12 * np.abs(S.rho()).max()
+    S.step(1)
+    assert not np.array_equal(S.get_particles(p)["x"], p0["x"])
