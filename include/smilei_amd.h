@@ -381,6 +381,20 @@ int smi_sort_exchange(int ndim, const int n_cells[3], const int patch_size[3], c
                       int64_t n, double *d_x, double *d_y, double *d_z, double *d_px, double *d_py, double *d_pz,
                       double *d_w, int16_t *d_charge, int32_t *d_key, int32_t *d_patch_count,
                       int32_t *d_bin_index, void *stream);
+/* No counterpart in the reference.  smi_sort_exchange without any device
+ * allocation, for callers that cannot allocate (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the scratch lives
+ * in a static 64 MiB device array of the library, one per GPU, held until
+ * the process ends.  Same arguments, results and errors as
+ * smi_sort_exchange; returns SMI_ERR_CAPACITY when the call needs more
+ * scratch words than smi_sort_exchange_noalloc_max_words() (about
+ * 2.25 words per cell of the grid plus 5.1 per particle).  Calls that
+ * overlap on one GPU share the array and race. */
+int smi_sort_exchange_noalloc(int ndim, const int n_cells[3], const int patch_size[3], const double cell_length[3],
+                              int64_t n, double *d_x, double *d_y, double *d_z, double *d_px, double *d_py,
+                              double *d_pz, double *d_w, int16_t *d_charge, int32_t *d_key, int32_t *d_patch_count,
+                              int32_t *d_bin_index, void *stream);
+int64_t smi_sort_exchange_noalloc_max_words(void);
 
 #ifdef __cplusplus
 }

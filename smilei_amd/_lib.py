@@ -110,6 +110,9 @@ SIGNATURES = {
     "smi_exchange_b": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I3, _I3, _VP, _VP]),
     "smi_sort_exchange": (ctypes.c_int, [ctypes.c_int, _I3, _I3, _D3, ctypes.c_int64] + [_VP] * 7 + [_VP] * 4
                           + [_VP]),
+    "smi_sort_exchange_noalloc": (ctypes.c_int, [ctypes.c_int, _I3, _I3, _D3, ctypes.c_int64] + [_VP] * 7
+                                  + [_VP] * 4 + [_VP]),
+    "smi_sort_exchange_noalloc_max_words": (ctypes.c_int64, []),
 }
 
 _lib = None

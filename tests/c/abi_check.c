@@ -119,6 +119,7 @@ static int mode_symbols(void)
         (fn_t)smi_maxwell_ampere_2d, (fn_t)smi_maxwell_faraday_2d,
         (fn_t)smi_current_deposition_3d, (fn_t)smi_density_deposition_3d,
         (fn_t)smi_sum_j_ghosts, (fn_t)smi_exchange_b, (fn_t)smi_sort_exchange,
+        (fn_t)smi_sort_exchange_noalloc, (fn_t)smi_sort_exchange_noalloc_max_words,
     };
     size_t k;
     for (k = 0; k < sizeof fns / sizeof fns[0]; ++k)
