@@ -123,6 +123,12 @@ def make_sim(args, W, form):
     sim = sa.Simulation("3Dcartesian", args.order, W["shape"], W["patch"], W["dx"], W["dt"],
                         precision=prec, deposit=sa.DEPOSIT_ATOMIC, use_graph=True)
     sim.set_tuning(sa.TUNE_CELL_ORDER, 1 if args.cell_order == "canonical" else 0)
+    if args.threads:
+        sim.set_tuning(sa.TUNE_THREADS, args.threads)
+    if args.tile_x and form == "mixed":
+        sim.set_tuning(sa.TUNE_TILE_X, args.tile_x)
+    if args.layout >= 0:
+        sim.set_tuning(sa.TUNE_INTERLEAVE, args.layout)
     ncell = int(np.prod(W["shape"]))
     per_species = W["build"](sim) if "build" in W else []
     for k, (name, m, q, sig, ppc, dens, drift, radius) in enumerate(W["species"]):
@@ -332,6 +338,11 @@ def main():
     ap.add_argument("--forms", default="mixed,fp64", help="comma list of mixed,fp64; the first is the headline")
     ap.add_argument("--copy-only", action="store_true", help="run the device copy probe only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--threads", type=int, default=0, help="particle-kernel workgroup size (with --no-autotune)")
+    ap.add_argument("--layout", type=int, default=-1,
+                    help="cluster layout of the sort: 0 cell-major, 1 lane-, 2 pair-interleaved (with --no-autotune)")
+    ap.add_argument("--tile-x", type=int, default=0,
+                    help="mixed form: clusters per workgroup along x, 1 or 2 (with --no-autotune)")
     ap.add_argument("--order", type=int, default=2, choices=[2, 4], help="interpolation order")
     ap.add_argument("--accuracy", type=int, default=0,
                     help="instead of timing: run both forms N steps from the same seeds and print the mixed "

@@ -113,20 +113,24 @@ int smi_set_particles(smi_context *ctx, int species, int64_t n,
                       const double *w);
 /* Seeded in-HBM initialisation of a species: `ppc` particles per cell,
  * uniform random in each cell, Gaussian momenta of std `sigma` (units of
- * m_species*c) per component plus `drift` along x, weight `density*V/ppc`.
+ * m_species*c) per component, weight `density*V/ppc`.  A drift u_d =
+ * drift_ux = gamma_d*v_d along x Lorentz-boosts the drawn momenta as the
+ * reference's boostParticles does (ParticleCreator.cpp:755-801: Zenitani's
+ * volume-transform correction, then u_x' = g u_x + gamma u_d); cold
+ * particles get u_x = u_d exactly.
  * Cells whose centre lies outside the cylinder of radius `radius` (in
  * cells, around the (y,z) axis centre; <= 0 disables) get no particles. */
 int smi_init_species(smi_context *ctx, int species, int ppc, double density,
                      double sigma, double drift_ux, double radius, uint64_t seed);
 /* Same with isotropic Maxwell-Juttner momenta of temperature theta =
  * T / (m_species c^2) (momentum_initialization = "maxwell-juttner" of the
- * reference, ParticleCreator.cpp:1002, in the rest frame; drift_ux is added
- * to u_x), drawn by inverse transform from 4096 quantiles of |u| tabulated on
- * the host; theta = 0 gives cold particles. */
+ * reference, ParticleCreator.cpp:1002, in the rest frame, then boosted by
+ * drift_ux as above), drawn by inverse transform from 4096 quantiles of |u|
+ * tabulated on the host; theta = 0 gives cold particles. */
 int smi_init_species_mj(smi_context *ctx, int species, int ppc, double density,
                         double theta, double drift_ux, double radius, uint64_t seed);
 /* Particles at the positions of species `src` (weights times weight_scale),
- * fresh Maxwell-Juttner momenta of temperature theta (0: cold) plus
+ * fresh Maxwell-Juttner momenta of temperature theta (0: cold) boosted by
  * `drift_ux` along x: position_initialization = "<species name>" of the
  * reference (src/Particles/ParticleCreator.cpp:277-284). */
 int smi_init_species_like(smi_context *ctx, int species, int src, double weight_scale, double theta,
@@ -194,8 +198,12 @@ void *smi_stream(smi_context *ctx);
  * stores each cluster's particles lane-interleaved (64 columns of
  * consecutive cell-sorted ranks, stored row by row), so that the 64 lanes of
  * a wavefront load consecutive slots and hold particles of distinct cells;
- * 0 = cell-major storage (lanes share cells).  Changing it re-sorts the
- * current state; smi_get_particles returns cell-major order either way.
+ * 0 = cell-major storage (lanes share cells); 2 (mixed form) = pair-interleaved (two
+ * consecutive ranks in the adjacent lanes 2c, 2c+1, the 32 pairs of a row in
+ * distinct cells): the mixed kernel then adds the J-tile words of the two
+ * lanes of a pair that share a node across the lanes (DPP) and issues one
+ * LDS atomic per pair.  Changing it re-sorts the current state;
+ * smi_get_particles returns cell-major order either way.
  * SMI_TUNE_THREADS: workgroup size per 4^3
  * cluster, 64 | 128 | 256 | 512 (particles per thread = cluster particles /
  * threads).  SMI_TUNE_PERSISTENT (both forms): 0 = one workgroup per
@@ -212,6 +220,13 @@ void *smi_stream(smi_context *ctx);
  * rows padded by one 64-bit word (fewer bank conflicts of the tile atomics
  * on dense plasmas), 0 = unpadded. */
 #define SMI_TUNE_TILE 6
+/* SMI_TUNE_TILE_X (mixed form, order 2): the patch tile one workgroup
+ * runs, in 4^3 clusters along x: 1 (default) or 2 (an 8x4x4-cell tile: one
+ * field tile and one J tile for two clusters, fewer halo points loaded and
+ * flushed per particle, two 512-thread workgroups per CU).  2 needs patches
+ * of a multiple of 8 cells along x; the persistent form runs one cluster
+ * per workgroup whatever this knob says. */
+#define SMI_TUNE_TILE_X 7
 /* Not searched by the autotuner (it changes what smi_get_particles returns):
  * SMI_TUNE_CELL_ORDER 1 (default) keeps the canonical order of particles
  * inside a cell (direction of motion, then previous slot); 0 leaves it to
@@ -224,7 +239,8 @@ int smi_get_tuning(smi_context *ctx, int key, int *value);
 /* Project autotuner (no counterpart in the reference, whose GPU launch shapes
  * are fixed): times the particle-kernel launches of one step on the live
  * particle state for every lane mapping x launch form (mixed form) x
- * cluster order (SMI_TUNE_XCD) x workgroup size 128/256/512, `reps`
+ * patch-tile extent (SMI_TUNE_TILE_X) x cluster order (SMI_TUNE_XCD) x
+ * workgroup size 128/256/512, `reps`
  * launches each, and keeps the fastest; best_ms (may be
  * NULL) receives its time.  Leaves the simulation state as it was. */
 int smi_autotune(smi_context *ctx, int reps, float *best_ms);

@@ -37,7 +37,7 @@ struct SpeciesState {
     void *buf[2][7] = {};          // x y z ux uy uz w: FP64, or FP32 (offsets) in the mixed form
     uint32_t *keys[2] = {};        // cell keys of the particles of buffer b
     SortBufs sb{};
-    bool lay = false;              // sb.perm holds the lane-interleaved cluster layout
+    int lay = 0;                   // cluster layout of sb.perm (layout_slot: 0 cell-major, 1 lane-, 2 pair-interleaved)
     double *scratch = nullptr;     // 7*cap, deterministic mode
     SpeciesK k{};
     SpeciesKM km{};
@@ -96,12 +96,13 @@ struct smi_context {
     int64_t steps = 0;             // time steps done (time_dual of the next step = (steps + 1.5) dt)
     std::vector<int> moving;       // species pushed by the next step, in id order (launch descriptors)
     bool mixed = false;            // SMI_MIXED form
-    int interleave = 1;            // lane-interleaved cluster layout of the sort (tuning)
+    int interleave = 1;            // cluster layout of the sort: 0 cell-major, 1 lane-, 2 pair-interleaved (tuning)
     int threads = 256;             // workgroup size of the particle kernel (tuning)
     int persistent = 0;            // persistent work-list form of the particle kernel (tuning)
     int cell_order = 1;            // canonical order inside a cell (atomic deposit may drop it)
     int xcd = 0;                   // XCD-aware cluster order of the grid launch (tuning)
     int tile = 1;                  // J-tile shape of the mixed kernel: rows padded by one word (tuning)
+    int tile_x = 1;                // patch-tile extent of the mixed kernel: clusters along x per workgroup (tuning)
     int persistent_blocks = 0;     // its grid, from the occupancy of the current knobs
     int *d_work = nullptr;         // work list (one int per cluster) + head/count (2 ints)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -127,12 +128,12 @@ void check_ctx(const smi_context *c)
     if (!c) throw Error(SMI_ERR_INVALID, "null context");
 }
 
-// The tile-deposit kernels (mixed form, atomic FP64 form) read each cluster
-// in the lane-interleaved layout unless the tuning turned it off; the
-// deterministic deposit walks the cells in order and keeps cell-major order.
-bool want_interleaved(const smi_context *c)
+// Cluster layout the tile-deposit kernels (mixed form, atomic FP64 form)
+// read: the tuning's (lane-interleaved by default); the deterministic
+// deposit walks the cells in order and keeps cell-major order (0).
+int want_interleaved(const smi_context *c)
 {
-    return c->p.deposit != SMI_DEPOSIT_DETERMINISTIC && c->interleave;
+    return c->p.deposit != SMI_DEPOSIT_DETERMINISTIC ? c->interleave : 0;
 }
 
 // The particle kernel records each input particle's cell-major position
@@ -140,7 +141,7 @@ bool want_interleaved(const smi_context *c)
 // interleaved and the next sort keeps the canonical order.
 bool want_rank(const smi_context *c)
 {
-    return want_interleaved(c) && c->cell_order;
+    return want_interleaved(c) != 0 && c->cell_order;
 }
 
 // The particle kernel places its cluster's stayers in the step's sort
@@ -212,7 +213,7 @@ void upload_mix(smi_context *c)
             m.counts = s.sb.counts;
             m.rank = want_rank(c) ? s.sb.rank : nullptr;
             m.stay = want_placed(c) ? s.sb.stay : nullptr;
-            m.lay = want_interleaved(c) ? 1 : 0;
+            m.lay = want_interleaved(c);
             v.push_back(m);
         }
     SMI_HIP(hipMemcpyAsync(c->d_mix, v.data(), v.size() * sizeof(MixSpecies), hipMemcpyHostToDevice, c->st));
@@ -243,8 +244,10 @@ void enqueue_particles(smi_context *c, int a)
             const int n = std::min(kMaxMerged, ns - g0);
             double cr_max = 0.;
             for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[c->moving[q]].km.cr_max);
+            // the two-cluster tile has no persistent form: one cluster there
+            const int tx = mixed_tile_x_ok(g, c->tile_x, c->persistent != 0) ? c->tile_x : 1;
             launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->threads, c->xcd != 0, c->tile,
-                                   c->persistent ? c->persistent_blocks : 0, c->d_work,
+                                   tx, want_interleaved(c) == 2, c->persistent ? c->persistent_blocks : 0, c->d_work,
                                    c->d_work + g.npatches * g.clpp, c->st);
         }
     } else {
@@ -275,7 +278,7 @@ void upload_fp(smi_context *c)
             f.scratch = s.scratch;
             f.rank = want_rank(c) ? s.sb.rank : nullptr;
             f.stay = want_placed(c) ? s.sb.stay : nullptr;
-            f.lay = want_interleaved(c) ? 1 : 0;
+            f.lay = want_interleaved(c);
             f.cap = s.cap;
             v.push_back(f);
         }
@@ -288,7 +291,7 @@ void upload_fp(smi_context *c)
 // descriptors refreshed.
 void apply_layout(smi_context *c)
 {
-    const bool want = want_interleaved(c);
+    const int want = want_interleaved(c);
     for (auto &S : c->sp) {
         launch_relayout(c->g, S.keys[c->cur], S.n, S.sb, S.lay, want, c->st);
         S.lay = want;
@@ -1011,13 +1014,13 @@ int smi_get_particles(smi_context *c, int s, double *x, double *y, double *z, do
         SMI_HIP(hipMemcpy(keys.data(), S.keys[c->cur], n * 4, hipMemcpyDeviceToHost));
     }
     if (!c->mixed && S.lay && n) {
-        // back to cell-major order: rank p of a cluster sits at slot cluster_slot(p)
+        // back to cell-major order: rank p of a cluster sits at slot layout_slot(p)
         std::vector<int32_t> off(c->g.ncells + 1), lay(perm);
         SMI_HIP(hipMemcpy(off.data(), S.sb.cell_off, off.size() * 4, hipMemcpyDeviceToHost));
         const int cpc = c->g.cells_per_cl;
         for (int64_t cb = 0; cb < c->g.ncells; cb += cpc) {
             const int a = off[cb], m = off[cb + cpc] - a;
-            for (int p = 0; p < m; ++p) perm[a + p] = lay[a + cluster_slot(p, m)];
+            for (int p = 0; p < m; ++p) perm[a + p] = lay[a + layout_slot(S.lay, p, m)];
         }
     }
     if (!c->mixed) {
@@ -1192,7 +1195,10 @@ int smi_set_tuning(smi_context *c, int key, int value)
     SMI_TRY
     check_ctx(c);
     if (key == SMI_TUNE_INTERLEAVE) {
-        c->interleave = value != 0;
+        if (value < 0 || value > 2) throw Error(SMI_ERR_INVALID, "SMI_TUNE_INTERLEAVE must be 0, 1 or 2");
+        if (value == 2 && !c->mixed)
+            throw Error(SMI_ERR_INVALID, "SMI_TUNE_INTERLEAVE = 2 (pair-interleaved) is a mixed-form layout");
+        c->interleave = value;
     } else if (key == SMI_TUNE_THREADS) {
         if (value != 64 && value != 128 && value != 256 && value != 512)
             throw Error(SMI_ERR_INVALID, "SMI_TUNE_THREADS must be 64, 128, 256 or 512");
@@ -1209,6 +1215,12 @@ int smi_set_tuning(smi_context *c, int key, int value)
     } else if (key == SMI_TUNE_TILE) {
         if (value != 0 && value != 1) throw Error(SMI_ERR_INVALID, "SMI_TUNE_TILE must be 0 or 1");
         c->tile = value;
+    } else if (key == SMI_TUNE_TILE_X) {
+        if (value != 1 && value != 2) throw Error(SMI_ERR_INVALID, "SMI_TUNE_TILE_X must be 1 or 2");
+        if (value == 2 && !(c->mixed && mixed_tile_x_ok(c->g, 2, false)))
+            throw Error(SMI_ERR_INVALID, "SMI_TUNE_TILE_X = 2 needs the mixed form, order 2 and patches of a "
+                                         "multiple of 8 cells along x");
+        c->tile_x = value;
     } else {
         throw Error(SMI_ERR_INVALID, "unknown tuning key");
     }
@@ -1230,6 +1242,7 @@ int smi_get_tuning(smi_context *c, int key, int *value)
     else if (key == SMI_TUNE_PERSISTENT) *value = c->persistent;
     else if (key == SMI_TUNE_XCD) *value = c->xcd;
     else if (key == SMI_TUNE_TILE) *value = c->tile;
+    else if (key == SMI_TUNE_TILE_X) *value = c->tile_x;
     else if (key == SMI_TUNE_CELL_ORDER) *value = c->cell_order || c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
     else throw Error(SMI_ERR_INVALID, "unknown tuning key");
     return SMI_OK;
@@ -1246,6 +1259,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     SMI_HIP(hipStreamSynchronize(c->st));
     drop_graphs(c);
     // candidates: cluster layout (tile-deposit forms) x J-tile shape (mixed)
+    // x patch-tile extent (mixed, grid launch: one or two clusters along x)
     // x persistent form x cluster order x workgroup size; each timed on the live
     // particle state with the particle launches of one step.  The trial
     // launches only write the output ping-pong buffer (rewritten by the next
@@ -1253,12 +1267,14 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     // below) and, in the atomic
     // deposit, J (cleared at the start of every step).  A layout is tried
     // after re-sorting the current state into it.
-    const int maps[2] = {1, 0};
+    const int maps[3] = {1, 0, 2};
     const bool det = c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
-    const int nmap = det ? 1 : 2, npers = 2, ntile = c->mixed ? 2 : 1;
+    // the pair-interleaved layout pays in the mixed kernel only (its
+    // pre-summed pair deposits)
+    const int nmap = det ? 1 : (c->mixed ? 3 : 2), npers = 2, ntile = c->mixed ? 2 : 1;
     const int sizes[3] = {128, 256, 512};
     int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent, best_xcd = c->xcd;
-    int best_tile = c->tile;
+    int best_tile = c->tile, best_tx = c->tile_x;
     float best = 3.4e38f;
     for (int m = 0; m < nmap; ++m) {
     if (!det && c->interleave != maps[m]) {
@@ -1266,14 +1282,17 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
         apply_layout(c);
     }
     for (int tl = 0; tl < ntile; ++tl)
+    for (int tx = 1; tx <= 2; ++tx)
     for (int pm = 0; pm < npers; ++pm)
     for (int xo = 0; xo < 2; ++xo)
         for (int z = 0; z < 3; ++z) {
             if (sizes[z] > particle_max_threads(c->g, c->mixed)) continue;
             if (pm && xo) continue;  // the persistent form takes clusters from its work list
+            if (tx == 2 && !(c->mixed && mixed_tile_x_ok(c->g, 2, pm != 0))) continue;
             c->threads = sizes[z];
             c->persistent = pm;
             c->xcd = xo;
+            c->tile_x = tx;
             if (c->mixed) c->tile = 1 - tl;
             refresh_persistent(c);
             enqueue_particles(c, c->cur);  // warm-up
@@ -1291,6 +1310,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
                 best_pers = c->persistent;
                 best_xcd = c->xcd;
                 best_tile = c->tile;
+                best_tx = c->tile_x;
             }
         }
     }
@@ -1299,6 +1319,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     c->persistent = best_pers;
     c->xcd = best_xcd;
     c->tile = best_tile;
+    c->tile_x = best_tx;
     refresh_persistent(c);
     apply_layout(c);
     for (auto &s : c->sp) {
@@ -1389,6 +1410,8 @@ static void compute_rho(smi_context *c)
     launch_fill(c->rho, c->g.ncells, 0.0, c->st);
     for (auto &S : c->sp) {
         if (c->mixed) launch_rho_mixed(c->g, S.km, S.mview(c->cur), S.keys[c->cur], S.sb.perm, S.n, c->rho, c->st);
+        else if (c->p.deposit == SMI_DEPOSIT_DETERMINISTIC)
+            launch_rho_gather(c->g, S.k, S.view(c->cur), S.sb.perm, S.sb.cell_off, S.lay, c->rho, c->st);
         else launch_rho(c->g, S.k, S.view(c->cur), S.sb.perm, S.n, c->rho, c->st);
     }
 }

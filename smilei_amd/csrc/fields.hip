@@ -228,6 +228,84 @@ __global__ void k_rho(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict
         }
 }
 
+// Global key of cell c (periodic indices in [0, N)): patch, 4^ND cluster,
+// cell in the cluster (Cluster3D::CellKey, nvidiaParticles.cu:639-660).
+template <int ND>
+__device__ inline uint32_t global_cell_key(const Geom &g, const int *c)
+{
+    int pc[3] = {0, 0, 0}, l[3] = {0, 0, 0};
+    for (int d = 0; d < ND; ++d) {
+        pc[d] = c[d] / g.n[d];
+        l[d] = c[d] - pc[d] * g.n[d];
+    }
+    if constexpr (ND == 3) {
+        const int p = (pc[0] * g.npatch[1] + pc[1]) * g.npatch[2] + pc[2];
+        const int cl = ((l[0] >> 2) * g.ncl[1] + (l[1] >> 2)) * g.ncl[2] + (l[2] >> 2);
+        return (uint32_t)((int64_t)p * g.cpp + cl * 64 + 16 * (l[0] & 3) + 4 * (l[1] & 3) + (l[2] & 3));
+    } else {
+        const int p = pc[0] * g.npatch[1] + pc[1];
+        const int cl = (l[0] >> 2) * g.ncl[1] + (l[1] >> 2);
+        return (uint32_t)((int64_t)p * g.cpp + cl * 16 + 4 * (l[0] & 3) + (l[1] & 3));
+    }
+}
+
+// Charge density by gather, for the deterministic deposit mode: one thread
+// per grid node adds to rho[node] the contributions (same shape and product
+// order as k_rho) of the particles of the (ORDER+2)^ND cells whose stencils
+// can reach the node, cells in x, y, z order and particles in their sorted
+// order inside each cell.  The summation order is fixed, so the density of
+// a state (and the Poisson field of a namelist start) is reproducible bit
+// for bit, which the atomic scatter is not.  cell_off: the species' sorted
+// cell ranges (cell-major ranks; stored in the cluster layout lay).
+template <int ND, int ORDER>
+__global__ void k_rho_gather(Geom g, SpeciesK sp, PartPtrs A, const int32_t *__restrict__ perm,
+                             const int32_t *__restrict__ cell_off, int lay, double *__restrict__ rho)
+{
+    constexpr int H = ORDER / 2, W = 2 * H + 1, R = 2 * H + 2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ncells) return;
+    const Idx3 q = decode(g, t);
+    const int node[3] = {q.i, q.j, q.k};
+    const double *X[3] = {A.x, A.y, A.z};
+    const int cpc = g.cells_per_cl;
+    double acc = 0.;
+    for (int a = 0; a < R; ++a)
+        for (int b = 0; b < R; ++b)
+            for (int e = 0; e < (ND == 3 ? R : 1); ++e) {
+                // source cell: unwrapped index cu, periodic image cw
+                const int cu[3] = {node[0] - H - 1 + a, node[1] - H - 1 + b, ND == 3 ? node[2] - H - 1 + e : 0};
+                int cw[3] = {0, 0, 0};
+                for (int d = 0; d < ND; ++d) cw[d] = wrapi(cu[d], g.N[d]);
+                const uint32_t key = global_cell_key<ND>(g, cw);
+                const int r0 = cell_off[key], r1 = cell_off[key + 1];
+                if (r0 == r1) continue;
+                int cs = 0, cn = 0;
+                if (lay) {
+                    const uint32_t cb = key - key % (uint32_t)cpc;
+                    cs = cell_off[cb];
+                    cn = cell_off[cb + cpc] - cs;
+                }
+                for (int r = r0; r < r1; ++r) {
+                    const int i = perm[lay ? cs + layout_slot(lay, r - cs, cn) : r];
+                    double S[3][5];
+                    int o[3] = {0, 0, 0};
+                    bool reach = true;
+                    for (int d = 0; d < ND && reach; ++d) {
+                        const double pn = X[d][i] * g.dinv[d];
+                        const int ip = (int)round(pn);
+                        o[d] = node[d] - (ip + (cu[d] - cw[d]) - H);
+                        reach = o[d] >= 0 && o[d] < W;
+                        if (reach) spline<ORDER>(pn - (double)ip, S[d]);
+                    }
+                    if (!reach) continue;
+                    const double cwp = sp.cw_scale * A.w[i];
+                    if constexpr (ND == 3) acc += cwp * S[0][o[0]] * S[1][o[1]] * S[2][o[2]];
+                    else acc += cwp * S[0][o[0]] * S[1][o[1]];
+                }
+            }
+    rho[t] += acc;
+}
+
 // residual = div(E) - rho on primal nodes (dual g+1 sits at (g+1/2)dx).
 __global__ void k_gauss(Geom g, FieldSet f, double *__restrict__ r)
 {
@@ -312,6 +390,19 @@ void launch_rho(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int3
         hipLaunchKernelGGL((k_rho<3, 4>), dim3(nblk(n, 256)), dim3(256), 0, st, g, sp, A, perm, n, rho);
     else
         hipLaunchKernelGGL((k_rho<2, 2>), dim3(nblk(n, 256)), dim3(256), 0, st, g, sp, A, perm, n, rho);
+    SMI_LAUNCH_CHECK();
+}
+
+void launch_rho_gather(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
+                       const int32_t *cell_off, int lay, double *rho, hipStream_t st)
+{
+    const dim3 grid(nblk(g.ncells, 256));
+    if (g.nd == 3 && g.order == 2)
+        hipLaunchKernelGGL((k_rho_gather<3, 2>), grid, dim3(256), 0, st, g, sp, A, perm, cell_off, lay, rho);
+    else if (g.nd == 3 && g.order == 4)
+        hipLaunchKernelGGL((k_rho_gather<3, 4>), grid, dim3(256), 0, st, g, sp, A, perm, cell_off, lay, rho);
+    else
+        hipLaunchKernelGGL((k_rho_gather<2, 2>), grid, dim3(256), 0, st, g, sp, A, perm, cell_off, lay, rho);
     SMI_LAUNCH_CHECK();
 }
 

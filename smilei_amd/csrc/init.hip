@@ -33,7 +33,14 @@ __device__ inline double u01(uint64_t seed, uint64_t i, uint32_t k)
 
 // Momentum of particle i: with a quantile table uq[0..nq) of |u| (a
 // Maxwell-Juttner distribution tabulated on the host) an isotropic vector of
-// that length, otherwise Gaussian components of std sigma; plus drift on x.
+// that length, otherwise Gaussian components of std sigma.  A drift (u_d =
+// g V along x, g = sqrt(1 + u_d^2)) Lorentz-boosts that rest-frame momentum
+// as ParticleCreator's boostParticles does (ParticleCreator.cpp:755-801):
+// first Zenitani's volume-transform correction (a particle whose rest-frame
+// velocity v_x points against the boost is reflected along it with
+// probability -V v_x, so that the boosted sample follows the drifting
+// distribution and not the rest-frame one), then u_x' = g u_x + gamma u_d
+// (a cold particle gets exactly u_d).
 __device__ inline void draw_momentum(uint64_t seed, int64_t i, double sigma, const double *uq, int nq, double drift,
                                      double *u)
 {
@@ -52,7 +59,12 @@ __device__ inline void draw_momentum(uint64_t seed, int64_t i, double sigma, con
             u[d] = sigma * sqrt(-2.0 * log(r1)) * cos(6.283185307179586 * r2);
         }
     }
-    u[0] += drift;
+    if (drift != 0.0) {
+        const double gb = sqrt(1.0 + drift * drift), V = drift / gb;
+        const double gam = sqrt(1.0 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (-V * u[0] / gam > u01(seed, i, 9)) u[0] = -u[0];
+        u[0] = fma(gb, u[0], gam * drift);
+    }
 }
 
 // Particle i lies in cell cells[i / ppc] (all cells in order when cells is

@@ -49,7 +49,7 @@ struct FpSpecies {
     double *scratch;                 // 7*cap doubles (deterministic deposit)
     int32_t *rank;                   // or null: cell-major input position of the output slots that left their cluster
     uint8_t *stay;                   // or null: placed stayers per cell (SortBufs::stay)
-    int lay;                         // the input's clusters are stored lane-interleaved
+    int lay;                         // cluster layout of the input (layout_slot)
     int64_t cap;
 };
 // One block per 4-cell cluster of every patch runs the particles of the ns
@@ -122,6 +122,51 @@ __host__ __device__ inline int cluster_rank(int m, int n)
     const int row = m >> 6, col = m & 63;
     return col < full ? col * q + row : full * q + (col - full) * (q - 1) + row;
 }
+// Pair-interleaved storage: ranks 2j and 2j+1 share a row in the adjacent
+// lanes 2c, 2c+1, the pairs dealt into 32 pair columns the way cluster_slot
+// deals ranks into 64 columns (an odd last rank takes the last slot).
+// Consecutive ranks mostly lie in the same cell, so the two lanes of a pair
+// usually deposit onto the same tile words and can add their contributions
+// before one of them issues the atomic; the 32 pairs of a row lie about
+// n/32 ranks apart, in different cells.  cluster_rank_pair is the inverse.
+__host__ __device__ inline int cluster_slot_pair(int p, int n)
+{
+    const int m = n >> 1;
+    if (p >= 2 * m) return p;
+    const int j = p >> 1;
+    const int q = (m + 31) >> 5, full = m - 32 * (q - 1);
+    int col, row;
+    if (j < full * q) {
+        col = j / q;
+        row = j - col * q;
+    } else {
+        const int r = j - full * q;
+        const int k = r / (q - 1);
+        col = full + k;
+        row = r - k * (q - 1);
+    }
+    return row * 64 + 2 * col + (p & 1);
+}
+__host__ __device__ inline int cluster_rank_pair(int m, int n)
+{
+    const int np = n >> 1;
+    if (m >= 2 * np) return m;
+    const int q = (np + 31) >> 5, full = np - 32 * (q - 1);
+    const int row = m >> 6, col = (m & 63) >> 1;
+    const int j = col < full ? col * q + row : full * q + (col - full) * (q - 1) + row;
+    return 2 * j + (m & 1);
+}
+// Cluster layouts of the sort: 0 cell-major, 1 lane-interleaved
+// (cluster_slot), 2 pair-interleaved (cluster_slot_pair).  Slot of the
+// cell-major rank p of a cluster of n particles, and the inverse.
+__host__ __device__ inline int layout_slot(int lay, int p, int n)
+{
+    return lay == 1 ? cluster_slot(p, n) : (lay == 2 ? cluster_slot_pair(p, n) : p);
+}
+__host__ __device__ inline int layout_rank(int lay, int m, int n)
+{
+    return lay == 1 ? cluster_rank(m, n) : (lay == 2 ? cluster_rank_pair(m, n) : m);
+}
 
 struct SortBufs {
     int32_t *counts;   // ncells, kept at zero between calls
@@ -159,12 +204,12 @@ int64_t sort_bsum_size(int64_t ncells);
 // not in_cell_order): the keys and sb.stay carry the particle kernel's
 // placed stayers (SortBufs::stay).
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st,
-                 bool counted = false, bool in_cell_order = true, bool interleaved = false,
+                 bool counted = false, bool in_cell_order = true, int layout = 0,
                  const int32_t *tie = nullptr, bool placed = false);
 // Re-stores the current sorted order in another cluster layout (perm via
 // tmp); keys are those the order was sorted from.
-void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, bool from_interleaved,
-                     bool to_interleaved, hipStream_t st);
+void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, int from_layout,
+                     int to_layout, hipStream_t st);
 // Keys of freshly loaded particles (positions in A, slot order = input).
 void launch_bin_initial(const Geom &g, const PartPtrs &A, int64_t n, uint32_t *keys, hipStream_t st);
 
@@ -183,16 +228,22 @@ struct MixSpecies {
     int32_t *counts;                // destination histogram
     int32_t *rank;                  // or null: cell-major input position of the output slots that left their cluster
     uint8_t *stay;                  // or null: placed stayers per cell (SortBufs::stay)
-    int lay;                        // the input's clusters are stored lane-interleaved
+    int lay;                        // cluster layout of the input (layout_slot)
 };
 // Particle kernel of the mixed form over ns <= kMaxMerged species that share
 // one J tile per cluster; cr_max = the largest current contribution of those
 // species (sets the fixed-point unit of the tile).  persistent_blocks > 0
 // selects the persistent form: that many workgroups pull the non-empty
 // clusters from a work list (work: one int per cluster, wcount: 2 ints).
+// tile_x: clusters per workgroup along x (the patch-tile extent), 1 or 2.
+// paired: the species are stored pair-interleaved (layout 2): order-2
+// one-cluster tiles then pre-sum the tile words of same-node lane pairs.
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, int threads, bool xcd, int tile, int persistent_blocks,
-                            int *work, int *wcount, hipStream_t st);
+                            const JPtrs &J, int *err, int threads, bool xcd, int tile, int tile_x, bool paired,
+                            int persistent_blocks, int *work, int *wcount, hipStream_t st);
+// Whether the mixed kernel has a patch tile of tile_x clusters for this
+// geometry and launch form.
+bool mixed_tile_x_ok(const Geom &g, int tile_x, bool persistent);
 // Resident workgroups of the persistent mixed kernel on the current device.
 int mixed_persistent_blocks(const Geom &g, int threads, int tile);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);
@@ -239,6 +290,10 @@ void launch_ukin(const PartPtrs &A, const int32_t *perm, int64_t n, double *part
 void launch_wsum(const double *w, const float *wf, int64_t n, double *partials, int nblocks, hipStream_t st);
 void launch_rho(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm, int64_t n,
                 double *rho, hipStream_t st);
+// Same density added in a fixed order (one thread per node gathering from
+// the sorted cells; cell_off / perm / lay as the sort left them).
+void launch_rho_gather(const Geom &g, const SpeciesK &sp, const PartPtrs &A, const int32_t *perm,
+                       const int32_t *cell_off, int lay, double *rho, hipStream_t st);
 void launch_gauss(const Geom &g, const FieldSet &f, double *rho_to_residual, hipStream_t st);
 
 }  // namespace smi

@@ -253,7 +253,7 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
             // the canonical in-cell order at the next sort: only read for
             // the particles that leave their cluster (the sort computes it
             // for the others)
-            if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
+            if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);  // layouts 0 and 1 (FP64 form)
         }
         keys[jslot] = (tag << 27) | gkey;
         B.x[jslot] = xw[0];

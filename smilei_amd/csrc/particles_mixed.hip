@@ -43,9 +43,9 @@ template <int ORDER>
 __device__ inline void splinef(float d, float *w)
 {
     if constexpr (ORDER == 2) {
-        w[0] = 0.5f * fmaf(d, d - 1.f, 0.25f);
+        w[0] = fmaf(d, fmaf(0.5f, d, -0.5f), 0.125f);
         w[1] = fmaf(-d, d, 0.75f);
-        w[2] = 0.5f * fmaf(d, d + 1.f, 0.25f);
+        w[2] = w[0] + d;
     } else {
         const float d2 = d * d;
         // 1/384 (1-2d)^4 and friends, Horner form of Interpolator3D4Order.h:62-67
@@ -113,9 +113,9 @@ template <int ORDER>
 __device__ inline void splined(double d, double *w)
 {
     if constexpr (ORDER == 2) {
-        w[0] = 0.5 * fma(d, d - 1.0, 0.25);
+        w[0] = fma(d, fma(0.5, d, -0.5), 0.125);
         w[1] = fma(-d, d, 0.75);
-        w[2] = 0.5 * fma(d, d + 1.0, 0.25);
+        w[2] = w[0] + d;
     } else {
         const double d2 = d * d;
         const double a = 0.5 - d, b = 0.5 + d;
@@ -182,33 +182,64 @@ __device__ inline double fixed_decode(unsigned long long v, double quantum)
 // NA = ORDER (the two supports coincide); otherwise NT = ORDER+2,
 // NA = ORDER+1.  tile(c, i, j, k) gives the LDS word of component c at
 // window indices relative to the particle's stencil origin.
-template <int ORDER, int NT, int NA, class Tile>
+// The tile word of the partner lane (lane ^ 1) of a pair of the
+// pair-interleaved layout: DPP quad permutation [1,0,3,2] on both halves.
+__device__ inline unsigned long long pair_word(unsigned long long v)
+{
+    typedef unsigned u2_t __attribute__((ext_vector_type(2)));
+    const u2_t a = __builtin_bit_cast(u2_t, v);
+    u2_t b;
+    b.x = __builtin_amdgcn_mov_dpp(a.x, 0xB1, 0xF, 0xF, false);
+    b.y = __builtin_amdgcn_mov_dpp(a.y, 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(unsigned long long, b);
+}
+
+// PAIR: wavefront-level pre-reduction of the tile atomics.  pairing = 1 on
+// the even lane of a pair of lanes whose particles have the same stencil
+// (same node, both stayers): it adds its partner's word to its own (the
+// fixed-point words are integers, so the sum of two is exact: the low 48
+// bits of (C + k1) + (C + k2) are k1 + k2) and issues the one atomic;
+// pairing = 2 on the odd lane, which issues none; 0: own atomic.
+template <int ORDER, int NT, int NA, bool PAIR = false, class Tile>
 __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const double (&DS)[3][ORDER + 2],
-                                        const int *w0, const double *crq, Tile tile)
+                                        const int *w0, const double *crq, Tile tile, int pairing = 0)
 {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int bx = (c + 1) % 3, ex = (c + 2) % 3;
-        double P[NT], Q[NT];
+        double P[NT], Q[NT], A[NA];
 #pragma unroll
         for (int m = 0; m < NT; ++m) {
-            P[m] = crq[c] * fma(0.5, DS[ex][m], S0[ex][m]);
-            Q[m] = crq[c] * fma(1.0 / 3.0, DS[ex][m], 0.5 * S0[ex][m]);
+            P[m] = fma(0.5, DS[ex][m], S0[ex][m]);
+            Q[m] = fma(1.0 / 3.0, DS[ex][m], 0.5 * S0[ex][m]);
+        }
+        double ps = DS[c][0];
+        A[0] = -crq[c] * ps;
+#pragma unroll
+        for (int r = 1; r < NA; ++r) {
+            ps += DS[c][r];
+            A[r] = -crq[c] * ps;
         }
 #pragma unroll
         for (int a = 0; a < NT; ++a)
 #pragma unroll
             for (int b = 0; b < NT; ++b) {
                 const double t = fma(S0[bx][a], P[b], DS[bx][a] * Q[b]);
-                double acc = kRound;
 #pragma unroll
                 for (int r = 1; r <= NA; ++r) {
-                    acc = fma(-DS[c][r - 1], t, acc);
+                    const double acc = fma(A[r - 1], t, kRound);
                     int idx[3];
                     idx[c] = w0[c] + r;
                     idx[bx] = w0[bx] + a;
                     idx[ex] = w0[ex] + b;
-                    atomicAdd(tile(c, idx[0], idx[1], idx[2]), (unsigned long long)__double_as_longlong(acc));
+                    unsigned long long v = (unsigned long long)__double_as_longlong(acc);
+                    if constexpr (PAIR) {
+                        const unsigned long long pv = pair_word(v);
+                        if (pairing == 1) v += pv;
+                        if (pairing != 2) atomicAdd(tile(c, idx[0], idx[1], idx[2]), v);
+                    } else {
+                        atomicAdd(tile(c, idx[0], idx[1], idx[2]), v);
+                    }
                 }
             }
     }
@@ -226,14 +257,23 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
 // fewer bank conflicts of the 64-bit atomics, 3 % faster).  The node-changer
 // queue takes the LDS left in the budget that keeps four workgroups per CU
 // (order 2).
-template <int ORDER, int JPAD>
+template <int ORDER, int JPAD, int TX = 1>
 struct MixTile {
     static constexpr int H = ORDER / 2;
     static constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
-    static constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GT * GSX;
-    static constexpr int JSY = JT + JPAD, JSX = JT * JSY, JSC = JT * JSX;
-    static constexpr int kBudget = ORDER == 2 ? 40960 - 64 : 65536;
-    static constexpr int kFixed = 6 * GF * 4 + 3 * JSC * 8 + kMaxMerged * 64 * 4 + 16;
+    // x extents of a tile of TX clusters along x
+    static constexpr int GTX = 4 * TX + 2 * H + 2, JTX = 4 * TX + 2 * H + 3;
+    static constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GTX * GSX;
+    // padded shape: rows of JT + 1 words and x-planes of 104 (order 2) or
+    // 136 (order 4) words, the strides for which the 64 nodes of a cluster,
+    // at one common stencil offset (the lanes of one atomic instruction),
+    // spread evenly over the LDS banks (each 8-byte word address mod 32 hit
+    // by two lanes, the fewest possible for 64 lanes; with x-planes of
+    // JT * (JT + 1) words, three)
+    static constexpr int JSY = JT + JPAD, JSX = JPAD ? (ORDER == 2 ? 104 : 136) : JT * JSY, JSC = JTX * JSX;
+    // one cluster: four workgroups per CU (order 2); two clusters: two
+    static constexpr int kBudget = ORDER == 2 ? (TX == 1 ? 40960 : 81920) - 64 : 65536;
+    static constexpr int kFixed = 6 * GF * 4 + 3 * JSC * 8 + TX * kMaxMerged * 64 * 4 + 16;
     static constexpr int QMAX = ORDER == 2 ? 256 : 128;
     static constexpr int QFIT = (kBudget - kFixed) / 32 / 32 * 32;
     static constexpr int QCAP = QFIT < QMAX ? QFIT : QMAX;
@@ -243,11 +283,11 @@ struct MixTile {
 // Esirkepov deposit of one particle from its old and new in-cell deltas
 // (dold, dnew in [-1/2, 1/2)), node shifts sh and the tile coordinates o3 of
 // its stencil origin; NT/NA as esirkepov3_fixed.
-template <int ORDER, int NT, int NA, int JPAD>
+template <int ORDER, int NT, int NA, int JPAD, int TX, bool PAIR = false>
 __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, const float *dnew, const int *sh,
-                                     const int *o3, const double *crq)
+                                     const int *o3, const double *crq, int pairing = 0)
 {
-    using TL = MixTile<ORDER, JPAD>;
+    using TL = MixTile<ORDER, JPAD, TX>;
     double S0[3][ORDER + 2], DS[3][ORDER + 2];
     int w0[3];
     if constexpr (NT == ORDER + 1) {
@@ -271,22 +311,28 @@ __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, 
         for (int d = 0; d < 3; ++d) windowd<ORDER>((double)dold[d], (double)dnew[d], sh[d], S0[d], DS[d], w0[d]);
     }
     unsigned long long *sJb = sJ + o3[0] * TL::JSX + o3[1] * TL::JSY + o3[2];
-    esirkepov3_fixed<ORDER, NT, NA>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
+    esirkepov3_fixed<ORDER, NT, NA, PAIR>(S0, DS, w0, crq, [&](int c, int a, int b, int k) {
         return &sJb[c * TL::JSC + a * TL::JSX + b * TL::JSY + k];
-    });
+    }, pairing);
 }
 
+// A tile of TX clusters adjacent along x in one patch (TX = 1: one
+// cluster).  Tiles of a patch are numbered like its clusters with the x
+// index divided by TX.
+template <int TX>
 struct Cl {
-    int p, pc[3], G0[3], lc0[3];  // patch, its coordinates, first global / in-patch cell
-    int64_t gbase;
+    int p, pc[3], G0[3], lc0[3];  // patch, its coordinates, first global / in-patch cell of the tile
+    int64_t gbase[TX];            // first global cell key of each cluster
 };
 
-__device__ inline Cl cluster_of(const Geom &g, int b)
+template <int TX>
+__device__ inline Cl<TX> tile_of(const Geom &g, int b)
 {
-    Cl c;
-    c.p = b / g.clpp;
-    const int cl = b % g.clpp;
-    const int cc[3] = {cl / (g.ncl[1] * g.ncl[2]), (cl / g.ncl[2]) % g.ncl[1], cl % g.ncl[2]};
+    Cl<TX> c;
+    const int tpp = g.clpp / TX;
+    c.p = b / tpp;
+    const int t = b % tpp;
+    const int cc[3] = {t / (g.ncl[1] * g.ncl[2]) * TX, (t / g.ncl[2]) % g.ncl[1], t % g.ncl[2]};
     c.pc[0] = c.p / (g.npatch[1] * g.npatch[2]);
     c.pc[1] = (c.p / g.npatch[2]) % g.npatch[1];
     c.pc[2] = c.p % g.npatch[2];
@@ -294,28 +340,32 @@ __device__ inline Cl cluster_of(const Geom &g, int b)
         c.lc0[d] = 4 * cc[d];
         c.G0[d] = c.pc[d] * g.n[d] + c.lc0[d];
     }
-    c.gbase = (int64_t)c.p * g.cpp + (int64_t)cl * g.cells_per_cl;
+#pragma unroll
+    for (int k = 0; k < TX; ++k)
+        c.gbase[k] = (int64_t)c.p * g.cpp + (int64_t)(((cc[0] + k) * g.ncl[1] + cc[1]) * g.ncl[2] + cc[2]) * 64;
     return c;
 }
 
 // One workgroup runs the particles of every species of the launch
-// (specs[0..ns)) of the 4^3 cluster `cluster` against one field tile and
-// one J tile.  Uniform across the workgroup; returns at once for an empty
-// cluster.  Thread t takes the slots t, t + blockDim.x, ... of the
-// cluster's range of perm: a wavefront reads 64 consecutive slots, whose
-// particles lie in different cells when the sort stored the range in the
-// lane-interleaved layout (cluster_slot).
-template <int ORDER, int JPAD>
+// (specs[0..ns)) of the tile `tile` (TX 4^3 clusters adjacent along x)
+// against one field tile and one J tile.  Uniform across the workgroup;
+// returns at once for an empty tile.  Thread t takes the slots t,
+// t + blockDim.x, ... of the tile's ranges of perm (cluster after cluster):
+// a wavefront reads 64 consecutive slots, whose particles lie in different
+// cells when the sort stored the range in the lane-interleaved layout
+// (cluster_slot).
+template <int ORDER, int JPAD, int TX, bool PAIRK = false>
 __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict__ specs, int ns, double cr_max,
-                                     const GatherFields &F, const JPtrs &J, int *__restrict__ err, int cluster)
+                                     const GatherFields &F, const JPtrs &J, int *__restrict__ err, int tile)
 {
-    using TL = MixTile<ORDER, JPAD>;
+    using TL = MixTile<ORDER, JPAD, TX>;
     constexpr int H = TL::H, GO = TL::GO, GT = TL::GT, JO = TL::JO, JT = TL::JT;
+    constexpr int JTX = TL::JTX;
     constexpr int GSX = TL::GSX, GSY = TL::GSY, GF = TL::GF;
-    constexpr int GT3 = GT * GT * GT, JT3 = JT * JT * JT;
+    constexpr int GT3 = TL::GTX * GT * GT, JT3 = JTX * JT * JT;
     __shared__ float sF[6 * GF];
     __shared__ unsigned long long sJ[3 * TL::JSC];
-    __shared__ int sHist[kMaxMerged][64];
+    __shared__ int sHist[TX][kMaxMerged][64];
     // particles whose node changes on some axis: their deposit (the wider
     // stencil) is deferred to a second pass over this queue, so that the
     // wavefronts of the main loop run the narrow stencil with all lanes;
@@ -325,9 +375,11 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     __shared__ float sQ[QCAP][8];
     __shared__ int sQn;
 
-    const Cl cl = cluster_of(g, cluster);
+    const Cl<TX> cl = tile_of<TX>(g, tile);
     int total = 0;
-    for (int q = 0; q < ns; ++q) total += specs[q].cell_off[cl.gbase + 64] - specs[q].cell_off[cl.gbase];
+    for (int q = 0; q < ns; ++q)
+#pragma unroll
+        for (int k = 0; k < TX; ++k) total += specs[q].cell_off[cl.gbase[k] + 64] - specs[q].cell_off[cl.gbase[k]];
     if (total == 0) return;
 
     const double *src[6] = {F.Ex, F.Ey, F.Ez, F.Bx, F.By, F.Bz};
@@ -339,7 +391,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         for (int f = 0; f < 6; ++f) sF[f * GF + a * GSX + b * GSY + c] = (float)src[f][idx];
     }
     for (int t = threadIdx.x; t < 3 * TL::JSC; t += blockDim.x) sJ[t] = 0ull;
-    for (int t = threadIdx.x; t < kMaxMerged * 64; t += blockDim.x) sHist[t / 64][t % 64] = 0;
+    for (int t = threadIdx.x; t < TX * kMaxMerged * 64; t += blockDim.x) (&sHist[0][0][0])[t] = 0;
     if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
 
@@ -367,25 +419,40 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     const gptr<int32_t> rank = glb(S.rank);
     const bool stay = S.stay != nullptr;
     const int lay = S.lay;
-    const int s = S.cell_off[cl.gbase], e = S.cell_off[cl.gbase + 64];
-    const int N = e - s;
-    for (int rel = threadIdx.x; rel < N; rel += blockDim.x) {
+    // the species' range of each cluster of the tile, one after the other
+    int s0[TX], nk[TX], N = 0;
+#pragma unroll
+    for (int k = 0; k < TX; ++k) {
+        s0[k] = S.cell_off[cl.gbase[k]];
+        nk[k] = S.cell_off[cl.gbase[k] + 64] - s0[k];
+        N += nk[k];
+    }
+    for (int rel0 = threadIdx.x; rel0 < N; rel0 += blockDim.x) {
+        // cluster k of the tile and the position rel in its range
+        int k = 0, rel = rel0;
+        if constexpr (TX == 2) {
+            k = rel0 >= nk[0] ? 1 : 0;
+            rel = rel0 - (k ? nk[0] : 0);
+        }
+        const int s = TX == 2 && k ? s0[1] : s0[0];
+        const int64_t gbase = TX == 2 && k ? cl.gbase[TX - 1] : cl.gbase[0];
         const int jslot = s + rel;
         const int i = perm[jslot];
-        const int lc = (int)((int64_t)(keysA[i] & kMask) - cl.gbase);
+        const int lc = (int)((int64_t)(keysA[i] & kMask) - gbase);
         float o[3] = {A.ox[i], A.oy[i], A.oz[i]};
         float u[3] = {A.ux[i], A.uy[i], A.uz[i]};
         const float w = A.w[i];
         if (lc < 0 || lc >= 64) {  // not in this cluster: corrupt state
             atomicOr(err, 1);
-            keysB[jslot] = ((stay ? kPlaceAtomic : 13u) << 27) | (uint32_t)cl.gbase;
+            keysB[jslot] = ((stay ? kPlaceAtomic : 13u) << 27) | (uint32_t)gbase;
             B.ox[jslot] = o[0]; B.oy[jslot] = o[1]; B.oz[jslot] = o[2];
             B.ux[jslot] = u[0]; B.uy[jslot] = u[1]; B.uz[jslot] = u[2];
             B.w[jslot] = w;
-            atomicAdd(&counts[cl.gbase], 1);
+            atomicAdd(&counts[gbase], 1);
             continue;
         }
-        const int l[3] = {lc >> 4, (lc >> 2) & 3, lc & 3};  // node in the cluster
+        // node in the tile (x: the nodes of the tile's cluster k start at 4k)
+        const int l[3] = {(lc >> 4) + 4 * k, (lc >> 2) & 3, lc & 3};
 
         // ---- shape factors and gather.  Primal stencil centred on the node;
         // the nearest dual node (global index g at (g - 1/2) dx) is l+1 for a
@@ -426,7 +493,10 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 #pragma unroll
         for (int d = 0; d < 3; ++d) t[d] = qmh * ig * Bf[d];
         const float t2 = fmaf(t[0], t[0], fmaf(t[1], t[1], t[2] * t[2]));
-        const float sfac = 2.f / (1.f + t2);
+        const float den = 1.f + t2;
+        float rd = __builtin_amdgcn_rcpf(den);
+        rd = fmaf(rd, fmaf(-den, rd, 1.f), rd);
+        const float sfac = 2.f * rd;
         const float up[3] = {fmaf(um[1], t[2], fmaf(-um[2], t[1], um[0])), fmaf(um[2], t[0], fmaf(-um[0], t[2], um[1])),
                              fmaf(um[0], t[1], fmaf(-um[1], t[0], um[2]))};
         const float sv[3] = {sfac * t[0], sfac * t[1], sfac * t[2]};
@@ -479,7 +549,22 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             // contributions in units of the cluster's fixed-point quantum
             const double cwq = sp.cwd * (double)w * inv_quantum;
             const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
-            if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER, JPAD>(sJ, dold, dnew, sh, o3, crq);
+            if constexpr (PAIRK) {
+                // pair-interleaved layout (the kernel is launched for it):
+                // pair the lanes 2c, 2c+1 whose particles are stayers of the
+                // same node (both lanes active here; the partner's value is
+                // only used when its lane is live)
+                int pairing = 0;
+                const uint64_t live = __ballot(1);
+                const int lane = threadIdx.x & 63;
+                const int me = moved == 0 ? (o3[0] | (o3[1] << 4) | (o3[2] << 8)) : -1;
+                const int other = __builtin_amdgcn_mov_dpp(me, 0xB1, 0xF, 0xF, false);
+                if (((live >> (lane ^ 1)) & 1ull) && me >= 0 && other == me) pairing = (lane & 1) ? 2 : 1;
+                if (moved == 0)
+                    deposit_fixed<ORDER, ORDER + 1, ORDER, JPAD, TX, true>(sJ, dold, dnew, sh, o3, crq, pairing);
+            } else {
+                if (moved == 0) deposit_fixed<ORDER, ORDER + 1, ORDER, JPAD, TX>(sJ, dold, dnew, sh, o3, crq);
+            }
             // node changers: one queue slot range per wavefront
             const uint64_t gm = __ballot(moved != 0);
             if (gm) {
@@ -502,27 +587,26 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
                         sQ[pos][6] = w;
                         sQ[pos][7] = __int_as_float(packed);
                     } else {
-                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD>(sJ, dold, dnew, sh, o3, crq);
+                        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD, TX>(sJ, dold, dnew, sh, o3, crq);
                     }
                 }
             }
         }
 
         // ---- destination key, direction of motion, store
-        const int dp = (dpc[0] * g.npatch[1] + dpc[1]) * g.npatch[2] + dpc[2];
-        const int dcl = ((lp[0] >> 2) * g.ncl[1] + (lp[1] >> 2)) * g.ncl[2] + (lp[2] >> 2);
-        const uint32_t gkey =
-            (uint32_t)((int64_t)dp * g.cpp + dcl * 64 + 16 * (lp[0] & 3) + 4 * (lp[1] & 3) + (lp[2] & 3));
+        const uint32_t dp = __umul24(__umul24(dpc[0], g.npatch[1]) + dpc[1], g.npatch[2]) + dpc[2];
+        const uint32_t dcl = __umul24(__umul24(lp[0] >> 2, g.ncl[1]) + (lp[1] >> 2), g.ncl[2]) + (lp[2] >> 2);
+        const uint32_t gkey = dp * (uint32_t)g.cpp + dcl * 64 + 16 * (lp[0] & 3) + 4 * (lp[1] & 3) + (lp[2] & 3);
         // top key bits: the direction of motion, or with `stay` the rank
         // among this cluster's stayers of the cell (placed by the sort)
         uint32_t tag = (uint32_t)((mdir[0] + 1) * 9 + (mdir[1] + 1) * 3 + (mdir[2] + 1));
-        const int64_t rc = (int64_t)gkey - cl.gbase;
-        if (rc >= 0 && rc < 64) {
+        const uint32_t rc = gkey - (uint32_t)gbase;
+        if (rc < 64) {
             if (stay) {
-                const int r = atomicAdd(&sHist[sid][rc], 1);
+                const int r = atomicAdd(&sHist[k][sid][rc], 1);
                 tag = r < (int)kPlaceMax ? (uint32_t)r : kPlaceAtomic;
             } else {
-                atomicAdd(&sHist[sid][rc], 1);
+                atomicAdd(&sHist[k][sid][rc], 1);
             }
         } else {
             atomicAdd(&counts[gkey], 1);
@@ -531,7 +615,11 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             // the canonical in-cell order at the next sort: only read for
             // the particles that leave their cluster (the sort computes it
             // for the others)
-            if (rank) rank[jslot] = s + (lay ? cluster_rank(rel, N) : rel);
+            if (rank) {
+                // (the pair-interleaved layout runs the PAIRK kernel)
+                const int nc = TX == 2 && k ? nk[TX - 1] : nk[0];
+                rank[jslot] = s + (PAIRK ? cluster_rank_pair(rel, nc) : (lay ? cluster_rank(rel, nc) : rel));
+            }
         }
         keysB[jslot] = (tag << 27) | gkey;
         B.ox[jslot] = on[0];
@@ -545,12 +633,14 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     }  // species
 
     __syncthreads();
-    for (int t = threadIdx.x; t < ns * 64; t += blockDim.x) {
-        const int c = sHist[t / 64][t % 64];
+    for (int t = threadIdx.x; t < TX * ns * 64; t += blockDim.x) {
+        const int k = t / (ns * 64), q = (t / 64) % ns, cell = t % 64;
+        const int c = sHist[k][q][cell];
         if (c) {
-            const MixSpecies &S = specs[t / 64];
-            atomicAdd(&S.counts[cl.gbase + t % 64], c);
-            if (S.stay) S.stay[cl.gbase + t % 64] = (uint8_t)(c < (int)kPlaceMax ? c : (int)kPlaceMax);
+            const MixSpecies &S = specs[q];
+            const int64_t gb = TX == 2 && k ? cl.gbase[TX - 1] : cl.gbase[0];
+            atomicAdd(&S.counts[gb + cell], c);
+            if (S.stay) S.stay[gb + cell] = (uint8_t)(c < (int)kPlaceMax ? c : (int)kPlaceMax);
         }
     }
     // deferred deposits of the node changers
@@ -564,13 +654,13 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         const int packed = __float_as_int(sQ[t][7]);
         const int o3[3] = {packed & 7, (packed >> 3) & 7, (packed >> 6) & 7};
         const int sh[3] = {((packed >> 9) & 3) - 1, ((packed >> 11) & 3) - 1, ((packed >> 13) & 3) - 1};
-        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD>(sJ, dold, dnew, sh, o3, crq);
+        deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD, TX>(sJ, dold, dnew, sh, o3, crq);
     }
     __syncthreads();
     double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
     for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
         const int c = t / JT3, r = t % JT3;
-        const int a = r / (JT * JT), b = (r / JT) % JT, k = r % JT;
+        const int a = r / (JT * JT), b = (r / JT) % JT, k = r % JT;  // a < JTX
         const double v = fixed_decode(sJ[c * TL::JSC + a * TL::JSX + b * TL::JSY + k], quantum);
         if (v == 0.) continue;
         const int gi = wrapi(cl.G0[0] - JO + a, g.N[0]);
@@ -580,16 +670,17 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     }
 }
 
-// One workgroup per cluster of the grid (xcd: XCD-aware cluster order).
-// Order 2: at most 128 VGPRs and 40 KB of LDS, so that four wavefronts per
-// SIMD are resident (four 256-thread workgroups per CU).
-template <int ORDER, int JPAD>
+// One workgroup per tile of TX clusters of the grid (xcd: XCD-aware tile
+// order).  Order 2: at most 128 VGPRs, and 40 KB (one cluster) or 80 KB (two)
+// of LDS, so that four wavefronts per SIMD are resident (four 256-thread or
+// two 512-thread workgroups per CU).
+template <int ORDER, int JPAD, int TX, bool PAIRK = false>
 __global__ void __launch_bounds__(512, ORDER == 2 ? 4 : 2)
 k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
                   int *__restrict__ err, int xcd)
 {
     const int b = xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x;
-    mixed_cluster<ORDER, JPAD>(g, specs, ns, cr_max, F, J, err, b);
+    mixed_cluster<ORDER, JPAD, TX, PAIRK>(g, specs, ns, cr_max, F, J, err, b);
 }
 
 // Persistent form: a grid sized to the resident workgroups pulls clusters
@@ -608,7 +699,7 @@ k_particles_mixed_list(Geom g, const MixSpecies *__restrict__ specs, int ns, dou
         __syncthreads();
         const int item = sItem;
         if (item >= nwork) break;
-        mixed_cluster<ORDER, JPAD>(g, specs, ns, cr_max, F, J, err, work[item]);
+        mixed_cluster<ORDER, JPAD, 1>(g, specs, ns, cr_max, F, J, err, work[item]);
     }
 }
 
@@ -620,8 +711,8 @@ __global__ void k_worklist(Geom g, const MixSpecies *__restrict__ specs, int ns,
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     int total = 0;
     if (c < g.npatches * g.clpp) {
-        const Cl cl = cluster_of(g, c);
-        for (int q = 0; q < ns; ++q) total += specs[q].cell_off[cl.gbase + 64] - specs[q].cell_off[cl.gbase];
+        const Cl<1> cl = tile_of<1>(g, c);
+        for (int q = 0; q < ns; ++q) total += specs[q].cell_off[cl.gbase[0] + 64] - specs[q].cell_off[cl.gbase[0]];
     }
     const uint64_t m = __ballot(total > 0);
     if (!m) return;
@@ -765,26 +856,48 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
+bool mixed_tile_x_ok(const Geom &g, int tile_x, bool persistent)
+{
+    if (tile_x == 1) return true;
+    // two clusters along x: order 2, an even number of clusters per patch
+    // along x, grid launch
+    return tile_x == 2 && g.order == 2 && g.ncl[0] % 2 == 0 && !persistent;
+}
+
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
-                            const JPtrs &J, int *err, int threads, bool xcd, int tile, int persistent_blocks,
-                            int *work, int *wcount, hipStream_t st)
+                            const JPtrs &J, int *err, int threads, bool xcd, int tile, int tile_x, bool paired,
+                            int persistent_blocks, int *work, int *wcount, hipStream_t st)
 {
     if (tile != 0 && tile != 1) throw Error(-1, "mixed tile shape must be 0 or 1");
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
     if (ns < 1 || ns > kMaxMerged) throw Error(-1, "bad species count of a merged particle launch");
     if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
         throw Error(-1, "particle-kernel workgroup size must be 64, 128, 256 or 512");
+    if (!mixed_tile_x_ok(g, tile_x, persistent_blocks > 0))
+        throw Error(-1, "patch tile of this extent not available for this geometry or launch form");
     const int nclusters = g.npatches * g.clpp;
     const dim3 block(threads);
     if (persistent_blocks <= 0) {
-#define SMI_PM(O, P) hipLaunchKernelGGL((k_particles_mixed<O, P>), dim3(nclusters), block, 0, st, g, specs, ns, \
-                                        cr_max, F, J, err, xcd ? 1 : 0)
+        const dim3 grid(nclusters / tile_x);
+#define SMI_PM(O, P, T) hipLaunchKernelGGL((k_particles_mixed<O, P, T>), grid, block, 0, st, g, specs, ns, \
+                                           cr_max, F, J, err, xcd ? 1 : 0)
         if (g.order == 2) {
-            if (tile) SMI_PM(2, 1);
-            else SMI_PM(2, 0);
+            if (tile_x == 2) {
+                if (tile) SMI_PM(2, 1, 2);
+                else SMI_PM(2, 0, 2);
+            } else if (paired) {
+                // pair pre-reduction of the tile atomics (one-cluster tiles)
+                if (tile) hipLaunchKernelGGL((k_particles_mixed<2, 1, 1, true>), grid, block, 0, st, g, specs, ns,
+                                             cr_max, F, J, err, xcd ? 1 : 0);
+                else hipLaunchKernelGGL((k_particles_mixed<2, 0, 1, true>), grid, block, 0, st, g, specs, ns,
+                                        cr_max, F, J, err, xcd ? 1 : 0);
+            } else {
+                if (tile) SMI_PM(2, 1, 1);
+                else SMI_PM(2, 0, 1);
+            }
         } else {
-            if (tile) SMI_PM(4, 1);
-            else SMI_PM(4, 0);
+            if (tile) SMI_PM(4, 1, 1);
+            else SMI_PM(4, 0, 1);
         }
 #undef SMI_PM
         SMI_LAUNCH_CHECK();

@@ -200,7 +200,7 @@ constexpr int kScatThreads = 256, kScatItems = 4, kScatTile = kScatThreads * kSc
 
 __global__ void __launch_bounds__(kScatThreads)
 k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
-          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc, int placed)
+          int32_t *__restrict__ counts, int32_t *__restrict__ perm, int cpc, int lay, int placed)
 {
     const int lane = threadIdx.x & 63;
     const int64_t i0 = (int64_t)dev::xcd_cluster(blockIdx.x, gridDim.x) * kScatTile +
@@ -235,7 +235,7 @@ k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restric
         if (cpc) {
             const uint32_t cb = gc & ~(uint32_t)(cpc - 1);
             const int s = cell_off[cb];
-            perm[s + cluster_slot(pos - s, cell_off[cb + cpc] - s)] = (int32_t)i;
+            perm[s + layout_slot(lay, pos - s, cell_off[cb + cpc] - s)] = (int32_t)i;
         } else {
             perm[pos] = (int32_t)i;
         }
@@ -270,6 +270,7 @@ struct TieSrc {
     const int32_t *tie;
     const int32_t *clus_old;
     int cpcell, ncl;
+    int lay;  // layout of the previous order's clusters
     int64_t n;
     int stay13;
     // previous range [cs, cs + cn) of the cluster of cell g (cn = 0: none)
@@ -290,7 +291,7 @@ __device__ inline uint64_t fix_key(const uint32_t *keys, const TieSrc &T, int32_
     if (T.tie) {
         const int r = v - cs;
         if (r >= 0 && r < cn) {
-            t = (uint32_t)(cs + cluster_rank(r, cn));
+            t = (uint32_t)(cs + layout_rank(T.lay, r, cn));
             if (T.stay13) return (13ull << 32) | t;
         } else {
             t = (uint32_t)T.tie[v];
@@ -300,7 +301,7 @@ __device__ inline uint64_t fix_key(const uint32_t *keys, const TieSrc &T, int32_
 }
 
 __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
-                             int32_t *src, int32_t *dst, int cpc, TieSrc T, int32_t *__restrict__ big,
+                             int32_t *src, int32_t *dst, int cpc, int lay, TieSrc T, int32_t *__restrict__ big,
                              int32_t *__restrict__ nbig)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,7 +315,7 @@ __global__ void k_bucket_fix(const uint32_t *__restrict__ keys, int64_t ncells, 
         cn = cell_off[cb + cpc] - cs;
     }
     // final place of the particle of cell-major position pos
-    auto put = [&](int pos, int32_t v) { dst[cpc ? cs + cluster_slot(pos - cs, cn) : pos] = v; };
+    auto put = [&](int pos, int32_t v) { dst[cpc ? cs + layout_slot(lay, pos - cs, cn) : pos] = v; };
     if (n < 2) {
         if (cpc && n == 1) put(a, src[a]);
         return;
@@ -388,7 +389,7 @@ __device__ inline int wave_append(bool want, int32_t *counter)
 // `big` for k_bucket_fix_big.
 __global__ void __launch_bounds__(256)
 k_bucket_fix_mid(const uint32_t *__restrict__ keys, int64_t ncells, const int32_t *__restrict__ cell_off,
-                 int32_t *src, int32_t *dst, int cpc, TieSrc T, int32_t *__restrict__ big,
+                 int32_t *src, int32_t *dst, int cpc, int lay, TieSrc T, int32_t *__restrict__ big,
                  int32_t *__restrict__ nbig)
 {
     const int lane = threadIdx.x & 63;
@@ -415,7 +416,7 @@ k_bucket_fix_mid(const uint32_t *__restrict__ keys, int64_t ncells, const int32_
         const uint64_t k = lane < n ? fix_key(keys, T, v, ocs, ocn) : ~0ull;
         int r = 0;
         for (int j = 0; j < n; ++j) r += __shfl(k, j, 64) < k ? 1 : 0;
-        if (lane < n) dst[cpc ? cs + cluster_slot(a + r - cs, cn) : a + r] = v;
+        if (lane < n) dst[cpc ? cs + layout_slot(lay, a + r - cs, cn) : a + r] = v;
     }
 }
 
@@ -434,7 +435,8 @@ constexpr int kBigBlocks = 1024;
 
 __global__ void __launch_bounds__(kBigThreads)
 k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ cell_off, int32_t *src, int32_t *dst,
-                 int32_t *scr, int cpc, TieSrc T, const int32_t *__restrict__ big, int32_t *__restrict__ nbig)
+                 int32_t *scr, int cpc, int lay, TieSrc T, const int32_t *__restrict__ big,
+                 int32_t *__restrict__ nbig)
 {
     __shared__ uint64_t sK[kBigLds];
     __shared__ int32_t sV[kBigLds];
@@ -455,7 +457,7 @@ k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ 
             cs = cell_off[cb];
             cn = cell_off[cb + cpc] - cs;
         }
-        auto put = [&](int pos, int32_t v) { dst[cpc ? cs + cluster_slot(pos - cs, cn) : pos] = v; };
+        auto put = [&](int pos, int32_t v) { dst[cpc ? cs + layout_slot(lay, pos - cs, cn) : pos] = v; };
         int ocs, ocn;
         T.range(g, ocs, ocn);
         if (n <= kBigLds) {
@@ -521,9 +523,9 @@ k_bucket_fix_big(const uint32_t *__restrict__ keys, const int32_t *__restrict__ 
     }
 }
 
-// Slot j of the current order, stored with layout `from`, moved to its slot
-// under layout `to` (0: cell-major, else the cells per cluster of the
-// interleaved layout); the cluster of the particle comes from its key.
+// Slot j of the current order, stored with cluster layout `from`, moved to
+// its slot under layout `to` (layout_slot); the cluster of the particle
+// comes from its key, cpc = cells per cluster.
 __global__ void k_relayout(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,
                            const int32_t *__restrict__ src, int32_t *__restrict__ dst, int cpc, int from, int to)
 {
@@ -532,8 +534,8 @@ __global__ void k_relayout(const uint32_t *__restrict__ keys, int64_t n, const i
     const int32_t v = src[j];
     const uint32_t cb = (keys[v] & kKeyMask) & ~(uint32_t)(cpc - 1);
     const int s = cell_off[cb], m = cell_off[cb + cpc] - s;
-    const int p = from ? cluster_rank((int)(j - s), m) : (int)(j - s);
-    dst[s + (to ? cluster_slot(p, m) : p)] = v;
+    const int p = layout_rank(from, (int)(j - s), m);
+    dst[s + layout_slot(to, p, m)] = v;
 }
 
 // Patch and cell key of loaded particles; all get direction 13 ("stay") so
@@ -573,7 +575,7 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 int64_t sort_bsum_size(int64_t ncells) { return (ncells + kScanTile - 1) / kScanTile + 1; }
 
 void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, hipStream_t st, bool counted,
-                 bool in_cell_order, bool interleaved, const int32_t *tie, bool placed)
+                 bool in_cell_order, int layout, const int32_t *tie, bool placed)
 {
     placed = placed && counted && !in_cell_order;
     const int64_t nc = g.ncells;
@@ -593,7 +595,7 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
     for (int d = 0; d < g.nd; ++d)
         if (g.npatch[d] * g.n[d] <= 4) stay13 = 0;
     const TieSrc T{tie, counted && tie && in_cell_order ? sb.clus_old : nullptr, g.cells_per_cl,
-                   (int)(nc / g.cells_per_cl), n, stay13};
+                   (int)(nc / g.cells_per_cl), layout, n, stay13};
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, st, sb.counts, nc,
                        (const int32_t *)sb.bsum, sb.cell_off, placed ? sb.stay : nullptr,
                        const_cast<int32_t *>(T.clus_old), g.cells_per_cl);
@@ -602,22 +604,22 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
         // cell-major places go straight to perm unless the canonical order
         // inside the cells and the interleaved layout are both asked for:
         // then they are ordered in tmp and laid out into perm
-        const int cpc = interleaved ? g.cells_per_cl : 0;
-        int32_t *first = in_cell_order && interleaved ? sb.tmp : sb.perm;
+        const int cpc = layout ? g.cells_per_cl : 0;
+        int32_t *first = in_cell_order && layout ? sb.tmp : sb.perm;
         hipLaunchKernelGGL(k_scatter, dim3(nblk(n, kScatTile)), dim3(kScatThreads), 0, st, keys, n, (const int32_t *)sb.cell_off,
-                           sb.counts, first, in_cell_order ? 0 : cpc, placed ? 1 : 0);
+                           sb.counts, first, in_cell_order ? 0 : cpc, layout, placed ? 1 : 0);
         SMI_LAUNCH_CHECK();
         if (in_cell_order) {
             if (sb.big) launch_fill_i32(sb.nbig, 2, 0, st);
             hipLaunchKernelGGL(k_bucket_fix, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, T, sb.big, sb.nbig);
+                               (const int32_t *)sb.cell_off, first, sb.perm, cpc, layout, T, sb.big, sb.nbig);
             SMI_LAUNCH_CHECK();
             if (sb.big) {
                 hipLaunchKernelGGL(k_bucket_fix_mid, dim3(nblk(nc, 256)), dim3(256), 0, st, keys, nc,
-                                   (const int32_t *)sb.cell_off, first, sb.perm, cpc, T, sb.big, sb.nbig);
+                                   (const int32_t *)sb.cell_off, first, sb.perm, cpc, layout, T, sb.big, sb.nbig);
                 SMI_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_bucket_fix_big, dim3(kBigBlocks), dim3(kBigThreads), 0, st, keys,
-                                   (const int32_t *)sb.cell_off, first, sb.perm, sb.tmp, cpc, T,
+                                   (const int32_t *)sb.cell_off, first, sb.perm, sb.tmp, cpc, layout, T,
                                    (const int32_t *)sb.big, sb.nbig);
                 SMI_LAUNCH_CHECK();
             }
@@ -625,13 +627,12 @@ void launch_sort(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, h
     }
 }
 
-void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, bool from_interleaved,
-                     bool to_interleaved, hipStream_t st)
+void launch_relayout(const Geom &g, const uint32_t *keys, int64_t n, SortBufs &sb, int from_layout,
+                     int to_layout, hipStream_t st)
 {
-    if (n == 0 || from_interleaved == to_interleaved) return;
+    if (n == 0 || from_layout == to_layout) return;
     hipLaunchKernelGGL(k_relayout, dim3(nblk(n, 256)), dim3(256), 0, st, keys, n, (const int32_t *)sb.cell_off,
-                       (const int32_t *)sb.perm, sb.tmp, g.cells_per_cl, from_interleaved ? 1 : 0,
-                       to_interleaved ? 1 : 0);
+                       (const int32_t *)sb.perm, sb.tmp, g.cells_per_cl, from_layout, to_layout);
     SMI_LAUNCH_CHECK();
     SMI_HIP(hipMemcpyAsync(sb.perm, sb.tmp, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
 }

@@ -47,7 +47,21 @@ def field_positions(sim, name):
 
 
 def evaluate(profile, x, y, z, shape):
-    v = profile(x, y, z) if callable(profile) else profile
+    """Values of a profile on the points (x, y, z).  Like the reference's
+    Profile (src/Profiles/Profile.cpp:199-260), a callable is first tried on
+    numpy arrays, and evaluated point by point with floats when it does not
+    accept them (e.g. a profile written with math.cos)."""
+    if not callable(profile):
+        v = profile
+    else:
+        try:
+            v = np.asarray(profile(x, y, z), dtype=np.float64)
+            if v.size not in (1, int(np.prod(shape))):
+                raise ValueError("profile returned %d values for %d points" % (v.size, int(np.prod(shape))))
+        except (TypeError, ValueError):
+            xb, yb, zb = np.broadcast_arrays(x, y, z)
+            v = np.array([float(profile(float(a), float(b), float(c)))
+                          for a, b, c in zip(xb.ravel(), yb.ravel(), zb.ravel())]).reshape(xb.shape)
     return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), shape))
 
 

@@ -161,15 +161,15 @@ def cell_multisets_agree(a, b, s, tol):
         assert np.abs(pa[k][ia] - pb[k][ib]).max() <= tol * scale, k
 
 
-@pytest.mark.parametrize("interleave", [1, 0])
-@pytest.mark.parametrize("precision", [sa.PLAIN_FP64, sa.MIXED])
+@pytest.mark.parametrize("precision,interleave", [(sa.PLAIN_FP64, 1), (sa.PLAIN_FP64, 0), (sa.MIXED, 1),
+                                                  (sa.MIXED, 0), (sa.MIXED, 2)])
 def test_placed_stayers_keep_the_cell_contents(precision, interleave):
     """In-cell order left to the sort (SMI_TUNE_CELL_ORDER 0): the particle
     kernel places the first 31 of its cluster's stayers of each cell (their
     rank in the key's top bits), the others and the particles arriving from
     other clusters take the atomic count-down.  With 3000 slow particles in
     one cell (thousands of stayers, most of them past the placed ranks), a
-    switch to the canonical order and back, in both cluster layouts, every
+    switch to the canonical order and back, in the three cluster layouts, every
     cell holds the same particles as in the canonical run."""
     p = dense_plasma(n_dense=3000, sigma=0.05)
     a, b = new(precision), new(precision)
@@ -210,23 +210,27 @@ def test_placed_stayers_2d():
 def test_canonical_order_independent_of_layout(shape, patch):
     """The canonical in-cell order (direction of motion, then the previous
     cell-major position) is the same with the clusters stored
-    lane-interleaved -- where the in-cell pass computes the tie-break and
+    lane-interleaved or pair-interleaved -- where the in-cell pass computes
+    the tie-break and
     the direction of the particles that stayed in their cluster instead of
     reading them -- as cell-major; also in a box of one 8-cell patch along
     z (two clusters), where particles wrap around the periodic box from one
     cluster into the other through their own patch's faces."""
     dx = (0.2, 0.2, 0.2)
-    sims = []
-    for lay in (1, 0):
-        sim = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx))
-        sim.set_tuning(sa.TUNE_INTERLEAVE, lay)
-        p = thermal(shape, dx, 12, 0.3, 21)
-        load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
-        sim.step(6)
-        sims.append(sim)
-    a, b = sims[0].get_particles(0), sims[1].get_particles(0)
-    for k in ("patch", "key"):
-        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
-    for k in ("x", "y", "z", "px", "py", "pz"):
-        scale = np.abs(b[k]).max()
-        assert np.abs(a[k] - b[k]).max() <= 1e-12 * scale, k
+    # FP64 form: cell-major against lane-interleaved; mixed form (which also
+    # has the pair-interleaved layout): lane- against pair-interleaved
+    for prec, lays, tol in ((sa.PLAIN_FP64, (0, 1), 1e-12), (sa.MIXED, (1, 2), 1e-6)):
+        sims = []
+        for lay in lays:
+            sim = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=prec)
+            sim.set_tuning(sa.TUNE_INTERLEAVE, lay)
+            p = thermal(shape, dx, 12, 0.3, 21)
+            load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
+            sim.step(6)
+            sims.append(sim)
+        a, b = sims[1].get_particles(0), sims[0].get_particles(0)
+        for k in ("patch", "key"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+        for k in ("x", "y", "z", "px", "py", "pz"):
+            scale = np.abs(b[k]).max()
+            assert np.abs(a[k] - b[k]).max() <= tol * scale, k

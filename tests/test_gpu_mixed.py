@@ -57,7 +57,7 @@ def test_mixed_bookkeeping_exact(order):
         np.testing.assert_array_equal(sim.patch_counts(s), np.bincount(p["patch"], minlength=sim.npatch))
 
 
-@pytest.mark.parametrize("interleave", [1, 0])
+@pytest.mark.parametrize("interleave", [1, 0, 2])
 def test_mixed_energies_and_gauss_vs_fp64(interleave):
     nsteps = 100
     m, _, _ = make(sa.MIXED, interleave=interleave)

@@ -65,9 +65,11 @@ def test_tuning_rejects_bad_values():
     with pytest.raises(RuntimeError, match="SMI_TUNE_THREADS"):
         sim.set_tuning(sa.TUNE_THREADS, 96)
     with pytest.raises(RuntimeError, match="unknown tuning key"):
-        sim.set_tuning(7, 1)
+        sim.set_tuning(99, 1)
     with pytest.raises(RuntimeError, match="SMI_TUNE_TILE"):
         sim.set_tuning(sa.TUNE_TILE, 2)
+    with pytest.raises(RuntimeError, match="SMI_TUNE_INTERLEAVE"):
+        sim.set_tuning(sa.TUNE_INTERLEAVE, 3)
 
 
 @pytest.mark.parametrize("precision", [sa.MIXED, sa.PLAIN_FP64])
@@ -150,6 +152,86 @@ def test_persistent_deterministic_fp64_is_bit_identical():
         np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
     for f in ("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"):
         np.testing.assert_array_equal(a.get_field(f), b.get_field(f), err_msg=f)
+
+
+@pytest.mark.parametrize("threads,xcd,pad", [(512, 0, 1), (256, 1, 0), (128, 0, 1)])
+def test_two_cluster_patch_tile_agrees(threads, xcd, pad):
+    """The 8x4x4-cell patch tile (two clusters per workgroup) against the
+    one-cluster tile: the first step reads the same fields, so particles and
+    keys are identical after it.  J differs by the rounding of each
+    contribution to its tile's fixed-point quantum (2^-46 N cr_max for a
+    tile of N particles: about 3e-11 cr_max here, over a few hundred
+    contributions per word) and by the FP64 flush order, far below the 1e-8
+    bound a misplaced tile would break.  Over six steps keys and counts stay
+    equal; the fields then differ at the FP32 level of the mixed form (the
+    J differences move FP32 offsets by an ulp here and there), within 1e-5."""
+    a, b = make(sa.MIXED), make(sa.MIXED)
+    for sim in (a, b):
+        sim.set_tuning(sa.TUNE_THREADS, threads)
+        sim.set_tuning(sa.TUNE_XCD, xcd)
+        sim.set_tuning(sa.TUNE_TILE, pad)
+    b.set_tuning(sa.TUNE_TILE_X, 2)
+    assert (a.get_tuning(sa.TUNE_TILE_X), b.get_tuning(sa.TUNE_TILE_X)) == (1, 2)
+    a.step(1)
+    b.step(1)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        for k in ("key", "x", "y", "z", "px", "py", "pz", "w"):
+            np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
+    for f in ("Jx", "Jy", "Jz"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-8 * np.abs(ref).max(), f
+    a.step(5)
+    b.step(5)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+    for f in ("Ex", "Jz", "By"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-5 * np.abs(ref).max(), f
+
+
+@pytest.mark.parametrize("threads,tile_x", [(256, 1), (512, 2)])
+def test_pair_interleaved_layout_pre_reduction_is_exact(threads, tile_x):
+    """Pair-interleaved clusters (SMI_TUNE_INTERLEAVE 2): the mixed kernel
+    adds the tile words of two lanes whose particles share a node before one
+    atomic.  The fixed-point words are integers, so the J tiles hold exactly
+    the same sums as with the lane-interleaved layout: after one step the
+    particles are identical and J differs only by the order of the FP64
+    flush (1e-13); over six steps keys and counts stay equal."""
+    a, b = make(sa.MIXED), make(sa.MIXED)
+    for sim in (a, b):
+        sim.set_tuning(sa.TUNE_THREADS, threads)
+        sim.set_tuning(sa.TUNE_TILE_X, tile_x)
+    b.set_tuning(sa.TUNE_INTERLEAVE, 2)
+    assert (a.get_tuning(sa.TUNE_INTERLEAVE), b.get_tuning(sa.TUNE_INTERLEAVE)) == (1, 2)
+    a.step(1)
+    b.step(1)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        for k in ("key", "x", "y", "z", "px", "py", "pz", "w"):
+            np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
+    for f in ("Jx", "Jy", "Jz"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-13 * np.abs(ref).max(), f
+    a.step(5)
+    b.step(5)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+    for f in ("Ex", "Jz", "By"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
+
+
+def test_two_cluster_patch_tile_refused_where_unavailable():
+    """The FP64 form has no two-cluster tile: the knob is refused."""
+    sim = make(sa.PLAIN_FP64)
+    with pytest.raises(RuntimeError, match="TILE_X"):
+        sim.set_tuning(sa.TUNE_TILE_X, 2)
+    assert sim.get_tuning(sa.TUNE_TILE_X) == 1
 
 
 def test_tile_shapes_agree():
