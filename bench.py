@@ -51,12 +51,12 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
-TRAFFIC_PER_PUSH = {"beam": {"mixed": 127.1, "fp64": 186.6}}
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 134.2, "fp64": 186.9}}
 # VALU instructions per push of the mixed particle kernel on the beam-plasma
 # workload (SQ_INSTS_VALU per launch / particles, rocprofv3 --pmc:
 # profiles/k_particles_mixed_beam1B_sq.md), and the chip's VALU issue peak:
 # 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD.
-VALU_INSTR_PER_PUSH = {"beam": {"mixed": 1146.0 / 64.0}}
+VALU_INSTR_PER_PUSH = {"beam": {"mixed": 1094.0 / 64.0}}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 # configured run length of each workload (steps)
 RUN_STEPS = {"beam": 300, "thermal": 200, "lwfa": 0}
@@ -129,6 +129,8 @@ def make_sim(args, W, form):
         sim.set_tuning(sa.TUNE_TILE_X, args.tile_x)
     if args.layout >= 0:
         sim.set_tuning(sa.TUNE_INTERLEAVE, args.layout)
+    if args.node_acc >= 0 and form == "mixed":
+        sim.set_tuning(sa.TUNE_NODE_ACC, args.node_acc)
     ncell = int(np.prod(W["shape"]))
     per_species = W["build"](sim) if "build" in W else []
     for k, (name, m, q, sig, ppc, dens, drift, radius) in enumerate(W["species"]):
@@ -343,6 +345,9 @@ def main():
                     help="cluster layout of the sort: 0 cell-major, 1 lane-, 2 pair-interleaved (with --no-autotune)")
     ap.add_argument("--tile-x", type=int, default=0,
                     help="mixed form: clusters per workgroup along x, 1 or 2 (with --no-autotune)")
+    ap.add_argument("--node-acc", type=int, default=-1,
+                    help="mixed form, order 2: 1 = stayers deposited node by node, 0 = by particle "
+                         "(with --no-autotune)")
     ap.add_argument("--order", type=int, default=2, choices=[2, 4], help="interpolation order")
     ap.add_argument("--accuracy", type=int, default=0,
                     help="instead of timing: run both forms N steps from the same seeds and print the mixed "

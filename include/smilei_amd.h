@@ -227,6 +227,14 @@ void *smi_stream(smi_context *ctx);
  * of a multiple of 8 cells along x; the persistent form runs one cluster
  * per workgroup whatever this knob says. */
 #define SMI_TUNE_TILE_X 7
+/* SMI_TUNE_NODE_ACC (mixed form, order 2): 1 = the particles that keep their
+ * node are staged per node in LDS, 256 at a time, and their J-tile words
+ * summed node by node in registers, one LDS atomic per node and word (three
+ * workgroups per CU for the staging); 0 (default) = one atomic per particle
+ * and word.  The same fixed-point words either way.  Runs with 256-thread
+ * one-cluster grid launches and the lane-interleaved or cell-major layouts;
+ * other launch shapes deposit by particle whatever this knob says. */
+#define SMI_TUNE_NODE_ACC 8
 /* Not searched by the autotuner (it changes what smi_get_particles returns):
  * SMI_TUNE_CELL_ORDER 1 (default) keeps the canonical order of particles
  * inside a cell (direction of motion, then previous slot); 0 leaves it to
@@ -239,6 +247,7 @@ int smi_get_tuning(smi_context *ctx, int key, int *value);
 /* Project autotuner (no counterpart in the reference, whose GPU launch shapes
  * are fixed): times the particle-kernel launches of one step on the live
  * particle state for every lane mapping x launch form (mixed form) x
+ * deposit by particle or node (SMI_TUNE_NODE_ACC) x
  * patch-tile extent (SMI_TUNE_TILE_X) x cluster order (SMI_TUNE_XCD) x
  * workgroup size 128/256/512, `reps`
  * launches each, and keeps the fastest; best_ms (may be

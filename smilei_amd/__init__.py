@@ -15,11 +15,12 @@ from ._lib import SmiParams, check
 
 __all__ = ["Simulation", "FIELDS", "PLAIN_FP64", "MIXED", "DEPOSIT_ATOMIC", "DEPOSIT_DETERMINISTIC",
            "TUNE_INTERLEAVE", "TUNE_THREADS", "TUNE_PERSISTENT", "TUNE_CELL_ORDER", "TUNE_XCD", "TUNE_TILE",
-           "TUNE_TILE_X"]
+           "TUNE_TILE_X", "TUNE_NODE_ACC"]
 
 PLAIN_FP64, MIXED = 0, 1
 DEPOSIT_ATOMIC, DEPOSIT_DETERMINISTIC = 0, 1
 TUNE_INTERLEAVE, TUNE_THREADS, TUNE_PERSISTENT, TUNE_CELL_ORDER, TUNE_XCD, TUNE_TILE, TUNE_TILE_X = 1, 2, 3, 4, 5, 6, 7
+TUNE_NODE_ACC = 8
 FIELDS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Bx_m", "By_m", "Bz_m", "Jx", "Jy", "Jz"]
 _FID = {n: i for i, n in enumerate(FIELDS)}
 _D = ctypes.POINTER(ctypes.c_double)
@@ -132,7 +133,8 @@ class Simulation:
         check(self._L.smi_autotune(self.h, int(reps), ctypes.byref(ms)))
         return dict(interleave=self.get_tuning(TUNE_INTERLEAVE), threads=self.get_tuning(TUNE_THREADS),
                     persistent=self.get_tuning(TUNE_PERSISTENT), xcd=self.get_tuning(TUNE_XCD),
-                    tile=self.get_tuning(TUNE_TILE), tile_x=self.get_tuning(TUNE_TILE_X), particles_ms=ms.value)
+                    tile=self.get_tuning(TUNE_TILE), tile_x=self.get_tuning(TUNE_TILE_X),
+                    node_acc=self.get_tuning(TUNE_NODE_ACC), particles_ms=ms.value)
 
     def init_species_mj(self, s, ppc, density, theta, drift_ux=0.0, radius=0.0, seed=0):
         """Maxwell-Juttner momenta of temperature theta = T / (m c^2)."""

@@ -103,6 +103,7 @@ struct smi_context {
     int xcd = 0;                   // XCD-aware cluster order of the grid launch (tuning)
     int tile = 1;                  // J-tile shape of the mixed kernel: rows padded by one word (tuning)
     int tile_x = 1;                // patch-tile extent of the mixed kernel: clusters along x per workgroup (tuning)
+    int node_acc = 0;              // mixed kernel: stayers' tile words summed node by node (tuning)
     int persistent_blocks = 0;     // its grid, from the occupancy of the current knobs
     int *d_work = nullptr;         // work list (one int per cluster) + head/count (2 ints)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -246,8 +247,12 @@ void enqueue_particles(smi_context *c, int a)
             for (int q = g0; q < g0 + n; ++q) cr_max = std::max(cr_max, c->sp[c->moving[q]].km.cr_max);
             // the two-cluster tile has no persistent form: one cluster there
             const int tx = mixed_tile_x_ok(g, c->tile_x, c->persistent != 0) ? c->tile_x : 1;
+            const bool paired = want_interleaved(c) == 2;
+            // the node-accumulated deposit where the launch allows it, else
+            // the per-particle one
+            const bool na = c->node_acc && mixed_node_acc_ok(g, c->threads, tx, paired, c->persistent != 0);
             launch_particles_mixed(g, c->d_mix + a * ns + g0, n, cr_max, GF, J, c->err, c->threads, c->xcd != 0, c->tile,
-                                   tx, want_interleaved(c) == 2, c->persistent ? c->persistent_blocks : 0, c->d_work,
+                                   tx, paired, na, c->persistent ? c->persistent_blocks : 0, c->d_work,
                                    c->d_work + g.npatches * g.clpp, c->st);
         }
     } else {
@@ -1221,6 +1226,11 @@ int smi_set_tuning(smi_context *c, int key, int value)
             throw Error(SMI_ERR_INVALID, "SMI_TUNE_TILE_X = 2 needs the mixed form, order 2 and patches of a "
                                          "multiple of 8 cells along x");
         c->tile_x = value;
+    } else if (key == SMI_TUNE_NODE_ACC) {
+        if (value != 0 && value != 1) throw Error(SMI_ERR_INVALID, "SMI_TUNE_NODE_ACC must be 0 or 1");
+        if (value == 1 && !(c->mixed && c->g.order == 2))
+            throw Error(SMI_ERR_INVALID, "SMI_TUNE_NODE_ACC = 1 needs the mixed form at order 2");
+        c->node_acc = value;
     } else {
         throw Error(SMI_ERR_INVALID, "unknown tuning key");
     }
@@ -1243,6 +1253,7 @@ int smi_get_tuning(smi_context *c, int key, int *value)
     else if (key == SMI_TUNE_XCD) *value = c->xcd;
     else if (key == SMI_TUNE_TILE) *value = c->tile;
     else if (key == SMI_TUNE_TILE_X) *value = c->tile_x;
+    else if (key == SMI_TUNE_NODE_ACC) *value = c->node_acc;
     else if (key == SMI_TUNE_CELL_ORDER) *value = c->cell_order || c->p.deposit == SMI_DEPOSIT_DETERMINISTIC;
     else throw Error(SMI_ERR_INVALID, "unknown tuning key");
     return SMI_OK;
@@ -1259,6 +1270,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     SMI_HIP(hipStreamSynchronize(c->st));
     drop_graphs(c);
     // candidates: cluster layout (tile-deposit forms) x J-tile shape (mixed)
+    // x deposit by particle or by node (mixed, order 2)
     // x patch-tile extent (mixed, grid launch: one or two clusters along x)
     // x persistent form x cluster order x workgroup size; each timed on the live
     // particle state with the particle launches of one step.  The trial
@@ -1274,7 +1286,9 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     const int nmap = det ? 1 : (c->mixed ? 3 : 2), npers = 2, ntile = c->mixed ? 2 : 1;
     const int sizes[3] = {128, 256, 512};
     int best_map = c->interleave, best_threads = c->threads, best_pers = c->persistent, best_xcd = c->xcd;
-    int best_tile = c->tile, best_tx = c->tile_x;
+    int best_tile = c->tile, best_tx = c->tile_x, best_na = c->node_acc;
+    // node-accumulated deposit: mixed form, order 2
+    const int nna = c->mixed && c->g.order == 2 ? 2 : 1;
     float best = 3.4e38f;
     for (int m = 0; m < nmap; ++m) {
     if (!det && c->interleave != maps[m]) {
@@ -1282,6 +1296,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
         apply_layout(c);
     }
     for (int tl = 0; tl < ntile; ++tl)
+    for (int na = 0; na < nna; ++na)
     for (int tx = 1; tx <= 2; ++tx)
     for (int pm = 0; pm < npers; ++pm)
     for (int xo = 0; xo < 2; ++xo)
@@ -1289,7 +1304,9 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
             if (sizes[z] > particle_max_threads(c->g, c->mixed)) continue;
             if (pm && xo) continue;  // the persistent form takes clusters from its work list
             if (tx == 2 && !(c->mixed && mixed_tile_x_ok(c->g, 2, pm != 0))) continue;
+            if (na && !mixed_node_acc_ok(c->g, sizes[z], tx, c->interleave == 2, pm != 0)) continue;
             c->threads = sizes[z];
+            c->node_acc = na;
             c->persistent = pm;
             c->xcd = xo;
             c->tile_x = tx;
@@ -1311,6 +1328,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
                 best_xcd = c->xcd;
                 best_tile = c->tile;
                 best_tx = c->tile_x;
+                best_na = c->node_acc;
             }
         }
     }
@@ -1320,6 +1338,7 @@ int smi_autotune(smi_context *c, int reps, float *best_ms)
     c->xcd = best_xcd;
     c->tile = best_tile;
     c->tile_x = best_tx;
+    c->node_acc = best_na;
     refresh_persistent(c);
     apply_layout(c);
     for (auto &s : c->sp) {

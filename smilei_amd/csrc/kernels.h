@@ -238,12 +238,17 @@ struct MixSpecies {
 // tile_x: clusters per workgroup along x (the patch-tile extent), 1 or 2.
 // paired: the species are stored pair-interleaved (layout 2): order-2
 // one-cluster tiles then pre-sum the tile words of same-node lane pairs.
+// node_acc: the particles that keep their node are staged per node and
+// their tile words summed node by node in registers (one atomic per node
+// and word; mixed_node_acc_ok).
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
                             const JPtrs &J, int *err, int threads, bool xcd, int tile, int tile_x, bool paired,
-                            int persistent_blocks, int *work, int *wcount, hipStream_t st);
+                            bool node_acc, int persistent_blocks, int *work, int *wcount, hipStream_t st);
 // Whether the mixed kernel has a patch tile of tile_x clusters for this
 // geometry and launch form.
 bool mixed_tile_x_ok(const Geom &g, int tile_x, bool persistent);
+// Whether the node-accumulated deposit runs with this geometry and launch.
+bool mixed_node_acc_ok(const Geom &g, int threads, int tile_x, bool paired, bool persistent);
 // Resident workgroups of the persistent mixed kernel on the current device.
 int mixed_persistent_blocks(const Geom &g, int threads, int tile);
 void launch_to_mixed(const Geom &g, const PartPtrs &X, int64_t n, const MixPtrs &M, uint32_t *keys, hipStream_t st);

@@ -38,7 +38,9 @@ __device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, c
                                      const int *ip, double cw)
 {
     using TL = Tile<ORDER>;
-    constexpr int H = TL::H, JT = TL::JT, JT3 = JT * JT * JT;
+    constexpr int H = TL::H, JT = TL::JT;
+    // J-tile strides: order-2 rows padded by one word (fp_cluster's layout)
+    constexpr int JSY = ORDER == 2 ? JT + 1 : JT, JSX = JT * JSY, JSC = JT * JSX;
     double S0[3][ORDER + 2], DS[3][ORDER + 2];
     int w0[3];
 #pragma unroll
@@ -48,9 +50,9 @@ __device__ inline void deposit3_tile(double *sJ, const Geom &g, const int *G0, c
     const int o0 = ip[0] - (H + 1) - (G0[0] - TL::JO);
     const int o1 = ip[1] - (H + 1) - (G0[1] - TL::JO);
     const int o2 = ip[2] - (H + 1) - (G0[2] - TL::JO);
-    double *sJb = sJ + (o0 * JT + o1) * JT + o2;
+    double *sJb = sJ + o0 * JSX + o1 * JSY + o2;
     esirkepov3d_win<ORDER, NT>(S0, DS, w0, cr, [&](int c, int a, int b, int k, double v) {
-        atomicAdd(&sJb[c * JT3 + (a * JT + b) * JT + k], v);
+        atomicAdd(&sJb[c * JSC + a * JSX + b * JSY + k], v);
     });
 }
 
@@ -75,8 +77,11 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
     // x-planes of cells a wavefront covers do not share LDS banks
     constexpr int GSX = ND == 3 ? (ORDER == 2 ? 68 : GT * GT) : GT, GF = ND == 3 ? GT * GSX : GT3;
     constexpr int JT3 = ND == 3 ? JT * JT * JT : JT * JT;
+    // 3D order 2: J rows padded by one word, so that the tile atomics of a
+    // wavefront meet fewer LDS bank conflicts (the strides of deposit3_tile)
+    constexpr int JSY = ND == 3 && ORDER == 2 ? JT + 1 : JT, JSX = JT * JSY, JSC = ND == 3 ? JT * JSX : JT3;
     __shared__ double sF[6 * GF];
-    __shared__ double sJ[DEPOSIT ? 3 * JT3 : 1];
+    __shared__ double sJ[DEPOSIT ? 3 * JSC : 1];
     __shared__ int sHist[kMaxMerged][64];  // destination-cell counts of this cluster's own cells
     // 3D deposit: particles whose node changes on some axis are deposited in
     // a second pass over this queue (old delta, new position in cells, charge
@@ -85,7 +90,7 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
     // workgroup's LDS (field and J tiles of 10^3 and 11^3 points) stays within
     // 80 KB and two workgroups fit a CU.
     constexpr bool QUEUE = DEPOSIT && ND == 3;
-    constexpr int QCAP = QUEUE ? (ORDER == 4 ? 12 : 128) : 1;
+    constexpr int QCAP = QUEUE ? (ORDER == 4 ? 12 : 96) : 1;
     __shared__ double sQ[QCAP][8];
     __shared__ int sQn;
 
@@ -117,7 +122,7 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
         for (int f = 0; f < 6; ++f) sF[f * GF + so] = src[f][idx];
     }
     if constexpr (DEPOSIT)
-        for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) sJ[t] = 0.;
+        for (int t = threadIdx.x; t < 3 * JSC; t += blockDim.x) sJ[t] = 0.;
     for (int t = threadIdx.x; t < kMaxMerged * 64; t += blockDim.x) sHist[t / 64][t % 64] = 0;
     if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
@@ -348,8 +353,6 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
     if constexpr (DEPOSIT) {
         double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
         for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
-            const double v = sJ[t];
-            if (v == 0.) continue;
             const int c = t / JT3, r = t % JT3;
             int a, b, k;
             if constexpr (ND == 3) {
@@ -361,6 +364,8 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
                 b = r % JT;
                 k = 0;
             }
+            const double v = ND == 3 ? sJ[c * JSC + a * JSX + b * JSY + k] : sJ[t];
+            if (v == 0.) continue;
             const int gi = wrapi(cl.G0[0] - TL::JO + a, g.N[0]);
             const int gj = wrapi(cl.G0[1] - TL::JO + b, g.N[1]);
             const int gk = ND == 3 ? wrapi(cl.G0[2] - TL::JO + k, g.N[2]) : 0;

@@ -257,7 +257,7 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
 // fewer bank conflicts of the 64-bit atomics, 3 % faster).  The node-changer
 // queue takes the LDS left in the budget that keeps four workgroups per CU
 // (order 2).
-template <int ORDER, int JPAD, int TX = 1>
+template <int ORDER, int JPAD, int TX = 1, bool NACC = false>
 struct MixTile {
     static constexpr int H = ORDER / 2;
     static constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
@@ -271,9 +271,16 @@ struct MixTile {
     // by two lanes, the fewest possible for 64 lanes; with x-planes of
     // JT * (JT + 1) words, three)
     static constexpr int JSY = JT + JPAD, JSX = JPAD ? (ORDER == 2 ? 104 : 136) : JT * JSY, JSC = JTX * JSX;
-    // one cluster: four workgroups per CU (order 2); two clusters: two
-    static constexpr int kBudget = ORDER == 2 ? (TX == 1 ? 40960 : 81920) - 64 : 65536;
-    static constexpr int kFixed = 6 * GF * 4 + 3 * JSC * 8 + TX * kMaxMerged * 64 * 4 + 16;
+    // node accumulation (NACC): per-node staging of the stayers of one
+    // 256-particle round, KST entries of 32 bytes per node, and two sets of
+    // per-node counters
+    static constexpr int KST = 4;
+    static constexpr int kStage = NACC ? KST * 64 * 32 + 2 * 64 * 4 : 0;
+    // one cluster: four workgroups per CU (order 2; three with the node
+    // staging); two clusters: two
+    static constexpr int kBudget =
+        ORDER == 2 ? (TX == 1 ? (NACC ? 53248 : 40960) : 81920) - 64 : 65536;
+    static constexpr int kFixed = 6 * GF * 4 + 3 * JSC * 8 + TX * kMaxMerged * 64 * 4 + 16 + kStage;
     static constexpr int QMAX = ORDER == 2 ? 256 : 128;
     static constexpr int QFIT = (kBudget - kFixed) / 32 / 32 * 32;
     static constexpr int QCAP = QFIT < QMAX ? QFIT : QMAX;
@@ -316,6 +323,89 @@ __device__ inline void deposit_fixed(unsigned long long *sJ, const float *dold, 
     }, pairing);
 }
 
+// Node accumulation (NACC): the stayers of one round of the workgroup
+// (blockDim.x consecutive slots of a species' cluster range) are staged per
+// node (old and new deltas, weight) and deposited node by node: the thread
+// of component C and node n sums the fixed-point words of its node's m
+// staged particles in registers and adds each of the NA x NT x NT words of
+// the node's stencil to the tile with one atomic, instead of one atomic per
+// particle and word.  The accumulator starts at the rounding constant and
+// each contribution is rounded onto it by one FMA, so that it ends at
+// 1.5*2^52 + sum(k), k the integers the per-particle deposit adds (the same
+// words, but where a product falls exactly half-way between two integers of
+// the quantum: round-half-even then depends on the accumulator's parity).
+// The contributions repeat the per-particle arithmetic (deposit_fixed with
+// NT = ORDER + 1, NA = ORDER) operation for operation.
+template <int ORDER, int JPAD, int C>
+__device__ inline void node_sum(unsigned long long *sJ, const float4 (*st)[64][2], int n, int m, double cwd,
+                                double inv_quantum, const double *dxdt)
+{
+    using TL = MixTile<ORDER, JPAD, 1>;
+    constexpr int NT = ORDER + 1, NA = ORDER, BX = (C + 1) % 3, EX = (C + 2) % 3;
+    double acc[NT][NT][NA];
+#pragma unroll
+    for (int a = 0; a < NT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int r = 0; r < NA; ++r) acc[a][b][r] = kRound;
+    for (int e = 0; e < m; ++e) {
+        const float4 p0 = st[e][n][0], p1 = st[e][n][1];
+        const float dold[3] = {p0.x, p0.y, p0.z}, dnew[3] = {p1.x, p1.y, p1.z};
+        double S0[3][NT], DS[3][NT];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double s0[ORDER + 1], s1[ORDER + 1];
+            splined<ORDER>((double)dold[d], s0);
+            splined<ORDER>((double)dnew[d], s1);
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                S0[d][q] = s0[q];
+                DS[d][q] = s1[q] - s0[q];
+            }
+        }
+        const double cwq = cwd * (double)p0.w * inv_quantum;
+        const double crq = cwq * dxdt[C];
+        double P[NT], Q[NT], A[NA];
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            P[q] = fma(0.5, DS[EX][q], S0[EX][q]);
+            Q[q] = fma(1.0 / 3.0, DS[EX][q], 0.5 * S0[EX][q]);
+        }
+        double ps = DS[C][0];
+        A[0] = -crq * ps;
+#pragma unroll
+        for (int r = 1; r < NA; ++r) {
+            ps += DS[C][r];
+            A[r] = -crq * ps;
+        }
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                const double t = fma(S0[BX][a], P[b], DS[BX][a] * Q[b]);
+#pragma unroll
+                for (int r = 0; r < NA; ++r) acc[a][b][r] = fma(A[r], t, acc[a][b][r]);
+            }
+    }
+    if (m == 0) return;
+    // the node's stencil origin: window entry 0 at tile offset 1 on every axis
+    unsigned long long *sJb = sJ + C * TL::JSC + (n >> 4) * TL::JSX + ((n >> 2) & 3) * TL::JSY + (n & 3);
+#pragma unroll
+    for (int a = 0; a < NT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+                int idx[3];
+                idx[C] = 2 + r;
+                idx[BX] = 1 + a;
+                idx[EX] = 1 + b;
+                atomicAdd(&sJb[idx[0] * TL::JSX + idx[1] * TL::JSY + idx[2]],
+                          (unsigned long long)__double_as_longlong(acc[a][b][r]));
+            }
+}
+
 // A tile of TX clusters adjacent along x in one patch (TX = 1: one
 // cluster).  Tiles of a patch are numbered like its clusters with the x
 // index divided by TX.
@@ -354,11 +444,12 @@ __device__ inline Cl<TX> tile_of(const Geom &g, int b)
 // a wavefront reads 64 consecutive slots, whose particles lie in different
 // cells when the sort stored the range in the lane-interleaved layout
 // (cluster_slot).
-template <int ORDER, int JPAD, int TX, bool PAIRK = false>
+template <int ORDER, int JPAD, int TX, bool PAIRK = false, bool NACC = false>
 __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict__ specs, int ns, double cr_max,
                                      const GatherFields &F, const JPtrs &J, int *__restrict__ err, int tile)
 {
-    using TL = MixTile<ORDER, JPAD, TX>;
+    static_assert(!NACC || (TX == 1 && !PAIRK), "node accumulation runs one-cluster tiles");
+    using TL = MixTile<ORDER, JPAD, TX, NACC>;
     constexpr int H = TL::H, GO = TL::GO, GT = TL::GT, JO = TL::JO, JT = TL::JT;
     constexpr int JTX = TL::JTX;
     constexpr int GSX = TL::GSX, GSY = TL::GSY, GF = TL::GF;
@@ -374,6 +465,14 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     constexpr int QCAP = TL::QCAP;
     __shared__ float sQ[QCAP][8];
     __shared__ int sQn;
+    // NACC: staged stayers of the current round per node (entry e of node n:
+    // old deltas and weight, new deltas) and their counts, two sets of
+    // counters used in alternate rounds (one is cleared while the other is
+    // read)
+    constexpr int KS = NACC ? TL::KST : 1, NN = NACC ? 64 : 1;
+    __shared__ float4 sStage[KS][NN][2];
+    __shared__ int sCnt[2][NN];
+    int par = 0;
 
     const Cl<TX> cl = tile_of<TX>(g, tile);
     int total = 0;
@@ -392,6 +491,8 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     }
     for (int t = threadIdx.x; t < 3 * TL::JSC; t += blockDim.x) sJ[t] = 0ull;
     for (int t = threadIdx.x; t < TX * kMaxMerged * 64; t += blockDim.x) (&sHist[0][0][0])[t] = 0;
+    if constexpr (NACC)
+        for (int t = threadIdx.x; t < 2 * NN; t += blockDim.x) (&sCnt[0][0])[t] = 0;
     if (threadIdx.x == 0) sQn = 0;
     __syncthreads();
 
@@ -427,7 +528,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         nk[k] = S.cell_off[cl.gbase[k] + 64] - s0[k];
         N += nk[k];
     }
-    for (int rel0 = threadIdx.x; rel0 < N; rel0 += blockDim.x) {
+    auto particle = [&](const int rel0) {
         // cluster k of the tile and the position rel in its range
         int k = 0, rel = rel0;
         if constexpr (TX == 2) {
@@ -449,7 +550,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             B.ux[jslot] = u[0]; B.uy[jslot] = u[1]; B.uz[jslot] = u[2];
             B.w[jslot] = w;
             atomicAdd(&counts[gbase], 1);
-            continue;
+            return;
         }
         // node in the tile (x: the nodes of the tile's cluster k start at 4k)
         const int l[3] = {(lc >> 4) + 4 * k, (lc >> 2) & 3, lc & 3};
@@ -549,7 +650,19 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             // contributions in units of the cluster's fixed-point quantum
             const double cwq = sp.cwd * (double)w * inv_quantum;
             const double crq[3] = {cwq * dxdt[0], cwq * dxdt[1], cwq * dxdt[2]};
-            if constexpr (PAIRK) {
+            if constexpr (NACC) {
+                // stayers staged for the node round; past the staging
+                // capacity of the node, deposited at once
+                if (moved == 0) {
+                    const int e = atomicAdd(&sCnt[par][lc], 1);
+                    if (e < KS) {
+                        sStage[e][lc][0] = make_float4(dold[0], dold[1], dold[2], w);
+                        sStage[e][lc][1] = make_float4(dnew[0], dnew[1], dnew[2], 0.f);
+                    } else {
+                        deposit_fixed<ORDER, ORDER + 1, ORDER, JPAD, TX>(sJ, dold, dnew, sh, o3, crq);
+                    }
+                }
+            } else if constexpr (PAIRK) {
                 // pair-interleaved layout (the kernel is launched for it):
                 // pair the lanes 2c, 2c+1 whose particles are stayers of the
                 // same node (both lanes active here; the partner's value is
@@ -629,6 +742,28 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         B.uy[jslot] = un[1];
         B.uz[jslot] = un[2];
         B.w[jslot] = w;
+    };
+    if constexpr (NACC) {
+        // rounds of blockDim.x slots, each followed by the node deposit of
+        // its staged stayers (threads 0..191: component tid / 64, node
+        // tid % 64; threads 192..255 clear the other counter set)
+        for (int base = 0; base < N; base += blockDim.x) {
+            if (base + (int)threadIdx.x < N) particle(base + (int)threadIdx.x);
+            __syncthreads();
+            const int c = threadIdx.x >> 6, n = threadIdx.x & 63;
+            if (c < 3) {
+                const int cnt = sCnt[par][n], m = cnt < KS ? cnt : KS;
+                if (c == 0) node_sum<ORDER, JPAD, 0>(sJ, sStage, n, m, sp.cwd, inv_quantum, dxdt);
+                else if (c == 1) node_sum<ORDER, JPAD, 1>(sJ, sStage, n, m, sp.cwd, inv_quantum, dxdt);
+                else node_sum<ORDER, JPAD, 2>(sJ, sStage, n, m, sp.cwd, inv_quantum, dxdt);
+            } else if (c == 3) {
+                sCnt[par ^ 1][n] = 0;
+            }
+            __syncthreads();
+            par ^= 1;
+        }
+    } else {
+        for (int rel0 = threadIdx.x; rel0 < N; rel0 += blockDim.x) particle(rel0);
     }
     }  // species
 
@@ -674,13 +809,13 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
 // order).  Order 2: at most 128 VGPRs, and 40 KB (one cluster) or 80 KB (two)
 // of LDS, so that four wavefronts per SIMD are resident (four 256-thread or
 // two 512-thread workgroups per CU).
-template <int ORDER, int JPAD, int TX, bool PAIRK = false>
-__global__ void __launch_bounds__(512, ORDER == 2 ? 4 : 2)
+template <int ORDER, int JPAD, int TX, bool PAIRK = false, bool NACC = false>
+__global__ void __launch_bounds__(512, NACC ? 3 : (ORDER == 2 ? 4 : 2))
 k_particles_mixed(Geom g, const MixSpecies *__restrict__ specs, int ns, double cr_max, GatherFields F, JPtrs J,
                   int *__restrict__ err, int xcd)
 {
     const int b = xcd ? xcd_cluster(blockIdx.x, gridDim.x) : blockIdx.x;
-    mixed_cluster<ORDER, JPAD, TX, PAIRK>(g, specs, ns, cr_max, F, J, err, b);
+    mixed_cluster<ORDER, JPAD, TX, PAIRK, NACC>(g, specs, ns, cr_max, F, J, err, b);
 }
 
 // Persistent form: a grid sized to the resident workgroups pulls clusters
@@ -864,9 +999,16 @@ bool mixed_tile_x_ok(const Geom &g, int tile_x, bool persistent)
     return tile_x == 2 && g.order == 2 && g.ncl[0] % 2 == 0 && !persistent;
 }
 
+bool mixed_node_acc_ok(const Geom &g, int threads, int tile_x, bool paired, bool persistent)
+{
+    // order 2, one-cluster tiles, the lane or cell-major layouts, 256-thread
+    // grid launch (three wavefronts sum the three components of 64 nodes)
+    return g.order == 2 && tile_x == 1 && !paired && threads == 256 && !persistent;
+}
+
 void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, double cr_max, const GatherFields &F,
                             const JPtrs &J, int *err, int threads, bool xcd, int tile, int tile_x, bool paired,
-                            int persistent_blocks, int *work, int *wcount, hipStream_t st)
+                            bool node_acc, int persistent_blocks, int *work, int *wcount, hipStream_t st)
 {
     if (tile != 0 && tile != 1) throw Error(-1, "mixed tile shape must be 0 or 1");
     if (g.nd != 3) throw Error(-1, "the mixed-precision form is implemented for 3Dcartesian");
@@ -875,6 +1017,8 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
         throw Error(-1, "particle-kernel workgroup size must be 64, 128, 256 or 512");
     if (!mixed_tile_x_ok(g, tile_x, persistent_blocks > 0))
         throw Error(-1, "patch tile of this extent not available for this geometry or launch form");
+    if (node_acc && !mixed_node_acc_ok(g, threads, tile_x, paired, persistent_blocks > 0))
+        throw Error(-1, "node-accumulated deposit not available for this geometry or launch form");
     const int nclusters = g.npatches * g.clpp;
     const dim3 block(threads);
     if (persistent_blocks <= 0) {
@@ -885,6 +1029,12 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
             if (tile_x == 2) {
                 if (tile) SMI_PM(2, 1, 2);
                 else SMI_PM(2, 0, 2);
+            } else if (node_acc) {
+                // stayers deposited node by node (one atomic per node and word)
+                if (tile) hipLaunchKernelGGL((k_particles_mixed<2, 1, 1, false, true>), grid, block, 0, st, g, specs,
+                                             ns, cr_max, F, J, err, xcd ? 1 : 0);
+                else hipLaunchKernelGGL((k_particles_mixed<2, 0, 1, false, true>), grid, block, 0, st, g, specs, ns,
+                                        cr_max, F, J, err, xcd ? 1 : 0);
             } else if (paired) {
                 // pair pre-reduction of the tile atomics (one-cluster tiles)
                 if (tile) hipLaunchKernelGGL((k_particles_mixed<2, 1, 1, true>), grid, block, 0, st, g, specs, ns,

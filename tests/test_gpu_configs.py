@@ -143,18 +143,37 @@ def test_order4_mixed_vs_fp64_1000_steps():
     assert err["gauss"] < 1e-6, err
 
 
+def perturbed(p, dx, shape, seed):
+    """Particles moved by up to half an FP32 rounding step: positions by
+    <= 2^-25 cell, momenta by a relative <= 2^-24 (uniform, seeded)."""
+    rng = np.random.default_rng(seed)
+    q = dict(p)
+    for d, k in enumerate("xyz"):
+        q[k] = np.mod(p[k] + dx[d] * 2.0 ** -25 * rng.uniform(-1.0, 1.0, len(p[k])), shape[d] * dx[d])
+    for k in ("px", "py", "pz"):
+        q[k] = p[k] * (1.0 + 2.0 ** -24 * rng.uniform(-1.0, 1.0, len(p[k])))
+    return q
+
+
 def test_beam_plasma_mixed_vs_fp64_1000_steps():
     """The beam-plasma configuration on a reduced 128x64x64 box (same dx, dt,
     32x16x16 patches, e-/p+ background 8 ppc each at sigma 0.02, cold e-
     beam gamma = 50, n = 0.1, 8 ppc within 16 cells of the axis: 9.2e6
-    particles), 1000 steps in both forms from identical initial particles
-    (the mixed form's FP32 state), and a second FP64 run with another launch
-    shape (128 threads, XCD-aware order: the same arithmetic, J summed in
-    another order).  The beam-plasma instability amplifies any difference,
-    and the second FP64 run measures that floor.  Asserted: total-energy
-    difference <= 1e-5 of the total, field and kinetic energy differences
-    <= 1e-5 of the total or within 4x the FP64-vs-FP64 floor, Gauss drift
-    <= 1e-6."""
+    particles), both forms from identical initial particles (the mixed
+    form's FP32 state).
+
+    After the configured 300 steps: field, kinetic and total energy
+    differences <= 1e-5 of the total energy, Gauss drift <= 1e-6.
+
+    After 1000 steps the beam instability has grown (field energy x14) and
+    amplifies any difference between two runs, however small: a second
+    mixed run whose initial particles are moved by half an FP32 rounding
+    step (positions 2^-25 cell, momenta 2^-24 relative) ends ~4e-6 of the
+    total energy away from the first.  Asserted then: total-energy
+    difference <= 1e-5 and Gauss drift <= 1e-6; field and kinetic energy
+    differences <= 1e-5 of the total, or within 4x the divergence of the
+    perturbed mixed run (rounding-level differences amplified by the
+    instability, not a systematic error of the mixed form)."""
     shape, patch, dx = (128, 64, 64), (32, 16, 16), (0.25, 0.25, 0.25)
     ub = float(np.sqrt(50.0 ** 2 - 1.0))
     parts = [("electron", 1.0, -1.0, thermal(shape, dx, 8, 0.02, 61)),
@@ -162,23 +181,28 @@ def test_beam_plasma_mixed_vs_fp64_1000_steps():
              ("beam", 1.0, -1.0, thermal(shape, dx, 8, 0.0, 63, density=0.1, drift_ux=ub, radius=16.0))]
     m = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=sa.MIXED)
     f = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=sa.PLAIN_FP64)
-    f2 = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=sa.PLAIN_FP64)
-    f2.set_tuning(sa.TUNE_THREADS, 128)
-    f2.set_tuning(sa.TUNE_XCD, 1)
-    for name, mass, q, p in parts:
+    mp = sa.Simulation("3Dcartesian", 2, shape, patch, dx, cfl_dt(dx), precision=sa.MIXED)
+    for k, (name, mass, q, p) in enumerate(parts):
         load(m, m.add_species(name, mass, q, len(p["x"])), p)
-        # identical initial conditions: the FP64 runs start from the mixed
+        # identical initial conditions: the FP64 run starts from the mixed
         # form's own (FP32-rounded) particles
         pm = m.get_particles(m.species.index(name))
         load(f, f.add_species(name, mass, q, len(p["x"])), pm)
-        load(f2, f2.add_species(name, mass, q, len(p["x"])), pm)
+        load(mp, mp.add_species(name, mass, q, len(p["x"])), perturbed(pm, dx, shape, 70 + k))
     gm0, rho0 = m.gauss(), np.abs(m.rho()).max()
-    for sim in (f, f2, m):
-        sim.step(1000)
+    for sim in (f, m):
+        sim.step(300)
     err = errors(m, f, gm0, rho0)
-    floor = errors(f2, f)
+    print("beam-plasma mixed vs fp64 after 300 steps:", err)
+    assert err["field"] < 1e-5 and err["kinetic"] < 1e-5 and err["total"] < 1e-5, err
+    assert err["gauss"] < 1e-6, err
+    for sim in (f, m):
+        sim.step(700)
+    mp.step(1000)
+    err = errors(m, f, gm0, rho0)
+    floor = errors(mp, m)
     print("beam-plasma mixed vs fp64 after 1000 steps:", err)
-    print("beam-plasma fp64 vs fp64 (launch shape) after 1000 steps:", floor)
+    print("beam-plasma mixed, perturbed by half an FP32 rounding step, vs mixed after 1000 steps:", floor)
     assert err["total"] < 1e-5, err
     for k in ("field", "kinetic"):
         assert err[k] < max(1e-5, 4 * floor[k]), (k, err, floor)

@@ -226,6 +226,61 @@ def test_pair_interleaved_layout_pre_reduction_is_exact(threads, tile_x):
         assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
 
 
+@pytest.mark.parametrize("layout,tile", [(1, 1), (0, 1), (1, 0)])
+def test_node_accumulated_deposit_agrees(layout, tile):
+    """Node-accumulated deposit (SMI_TUNE_NODE_ACC 1): the stayers of each
+    256-particle round are staged per node and their tile words summed in
+    registers, one atomic per node and word.  Same fixed-point words as the
+    per-particle deposit: after one step the particles are identical and J
+    differs only by the order of the FP64 flush (1e-13); over six steps keys
+    and counts stay equal.  The cell-major layout (0) puts up to 256
+    particles of few nodes in one round, past the four staging entries of a
+    node: those are deposited at once (the overflow path)."""
+    a, b = make(sa.MIXED), make(sa.MIXED)
+    for sim in (a, b):
+        sim.set_tuning(sa.TUNE_THREADS, 256)
+        sim.set_tuning(sa.TUNE_INTERLEAVE, layout)
+        sim.set_tuning(sa.TUNE_TILE, tile)
+    b.set_tuning(sa.TUNE_NODE_ACC, 1)
+    assert (a.get_tuning(sa.TUNE_NODE_ACC), b.get_tuning(sa.TUNE_NODE_ACC)) == (0, 1)
+    a.step(1)
+    b.step(1)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        for k in ("key", "x", "y", "z", "px", "py", "pz", "w"):
+            np.testing.assert_array_equal(pa[k], pb[k], err_msg=k)
+    for f in ("Jx", "Jy", "Jz"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-13 * np.abs(ref).max(), f
+    a.step(5)
+    b.step(5)
+    for s in range(2):
+        pa, pb = a.get_particles(s), b.get_particles(s)
+        np.testing.assert_array_equal(pa["key"], pb["key"])
+        np.testing.assert_array_equal(a.patch_counts(s), b.patch_counts(s))
+    for f in ("Ex", "Jz", "By"):
+        ref = a.get_field(f)
+        assert np.abs(b.get_field(f) - ref).max() <= 1e-10 * np.abs(ref).max(), f
+    np.testing.assert_allclose(b.gauss(), a.gauss(), rtol=0, atol=1e-12 * np.abs(a.rho()).max())
+
+
+def test_node_accumulated_deposit_refused_where_unavailable():
+    """The node-accumulated deposit is an order-2 mixed-form mode; other
+    launch shapes of the mixed form keep the per-particle deposit."""
+    sim = make(sa.PLAIN_FP64)
+    with pytest.raises(RuntimeError, match="NODE_ACC"):
+        sim.set_tuning(sa.TUNE_NODE_ACC, 1)
+    assert sim.get_tuning(sa.TUNE_NODE_ACC) == 0
+    a, b = make(sa.MIXED), make(sa.MIXED)
+    b.set_tuning(sa.TUNE_NODE_ACC, 1)
+    b.set_tuning(sa.TUNE_THREADS, 128)  # not a node-accumulation launch: per particle
+    a.set_tuning(sa.TUNE_THREADS, 128)
+    a.step(1)
+    b.step(1)
+    for s in range(2):
+        np.testing.assert_array_equal(a.get_particles(s)["x"], b.get_particles(s)["x"])
+
+
 def test_two_cluster_patch_tile_refused_where_unavailable():
     """The FP64 form has no two-cluster tile: the knob is refused."""
     sim = make(sa.PLAIN_FP64)
