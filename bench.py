@@ -51,7 +51,7 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
-TRAFFIC_PER_PUSH = {"beam": {"mixed": 134.2, "fp64": 186.9}}
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 134.3, "fp64": 156.4}}
 # VALU instructions per push of the mixed particle kernel on the beam-plasma
 # workload (SQ_INSTS_VALU per launch / particles, rocprofv3 --pmc:
 # profiles/k_particles_mixed_beam1B_sq.md), and the chip's VALU issue peak:

@@ -222,12 +222,15 @@ def test_namelist_solves_poisson_for_a_nonneutral_start():
     np.testing.assert_allclose(off.gauss(), -r, rtol=0, atol=1e-13 * np.abs(r).max())
 
 
-def test_namelist_run_matches_oracle_bit_exact():
+def test_namelist_run_matches_oracle_bit_exact(tmp_path):
     """A namelist-built simulation (parameters derived from the namelist:
     timestep from timestep_over_CFL, patches from number_of_patches, species
     from profiles and temperatures, E from Poisson, an external B profile)
     run in the deterministic FP64 form equals the oracle started from the
-    same particles and fields, bit for bit, after 20 steps."""
+    same particles and fields, bit for bit, after 20 steps; its scalars.txt
+    record (written and read back as happi reads it) holds the oracle's
+    per-field and per-species energies (DiagnosticScalar.cpp:494-522,
+    660-690) to the 10 printed digits."""
     from oracle import pyoracle as O
     from tests.test_gpu_configs import compare, copy_to_oracle
     nl = N.load(NONNEUTRAL % "")
@@ -241,3 +244,17 @@ def test_namelist_run_matches_oracle_bit_exact():
     sim.step(n)
     o.step(n)
     compare(sim, o, 2, exact=True)
+    names = N.scalar_names(sim.species)
+    vals, _ = N.scalar_values(sim, [1.0, -1.0], names, None)
+    out = N.ScalarFile(str(tmp_path / "scalars.txt"), names)
+    out.write(n * nl.main.timestep, vals)
+    out.close()
+    got, steps, v = happi_read(str(tmp_path / "scalars.txt"), nl.main.timestep)
+    assert got == ["time"] + names and steps == [n]
+    fe, en = o.field_energies(), o.energies()
+    for k, f in enumerate(("Ex", "Ey", "Ez", "Bx_m", "By_m", "Bz_m")):
+        np.testing.assert_allclose(v["Uelm_" + f][0], fe[k], rtol=1e-9, atol=1e-12 * en.sum(), err_msg=f)
+    for k, sp in enumerate(sim.species):
+        np.testing.assert_allclose(v["Ukin_" + sp][0], en[1 + k], rtol=1e-9, err_msg=sp)
+    np.testing.assert_allclose(v["Uelm"][0], en[0], rtol=1e-9)
+    np.testing.assert_allclose(v["Utot"][0], en.sum(), rtol=1e-9)
