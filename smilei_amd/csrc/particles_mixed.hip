@@ -247,12 +247,15 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
 
 // LDS tiles of one cluster of the mixed kernel.  Field tile: GT^3 points
 // per field from G0-GO, strides GSX, GSY, 1 (floats).  J tile: JT^3 words
-// per component from G0-JO (stencil rows [c0-H-1, c0+4+H]), strides JSX,
-// JSY, 1 and JSC between components (64-bit words).  The x-stride of 76
+// per component from G0-JO, strides JSX, JSY, 1 and JSC between components
+// (64-bit words).  A particle binned to node l (0..3 per axis, tile
+// coordinate l + JO) deposits on tile rows l + w0 + m with window entries
+// m < ORDER + 2 and w0 in {0, 1} (esirkepov3_fixed): rows 0 .. 5 + 2H, so JT
+// = 6 + 2H rows cover every deposit.  The x-stride of 76
 // floats (not 8*8 = 64, a multiple of the LDS bank count) spreads the four
 // x-planes of cells that the lanes of a wavefront cover over different
 // banks (measured on the order-2 kernel: x-strides of 68, 76, 84 equal,
-// 72 and 80 6 % slower).  JPAD = 1 pads the J rows by one word (a tile
+// 72 and 80 6 % slower).  JPAD = 1 pads the J rows by two words (a tile
 // shape chosen by the autotuner: on the beam-plasma the padded rows meet
 // fewer bank conflicts of the 64-bit atomics, 3 % faster).  The node-changer
 // queue takes the LDS left in the budget that keeps four workgroups per CU
@@ -260,17 +263,20 @@ __device__ inline void esirkepov3_fixed(const double (&S0)[3][ORDER + 2], const 
 template <int ORDER, int JPAD, int TX = 1, bool NACC = false>
 struct MixTile {
     static constexpr int H = ORDER / 2;
-    static constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 3;
+    static constexpr int GO = H, GT = 4 + 2 * H + 2, JO = H + 1, JT = 4 + 2 * H + 2;
     // x extents of a tile of TX clusters along x
-    static constexpr int GTX = 4 * TX + 2 * H + 2, JTX = 4 * TX + 2 * H + 3;
+    static constexpr int GTX = 4 * TX + 2 * H + 2, JTX = 4 * TX + 2 * H + 2;
     static constexpr int GSY = GT, GSX = ORDER == 2 ? 76 : GT * GT, GF = GTX * GSX;
-    // padded shape: rows of JT + 1 words and x-planes of 104 (order 2) or
+    // padded shape: rows of JT + 2 words and x-planes of 104 (order 2) or
     // 136 (order 4) words, the strides for which the 64 nodes of a cluster,
     // at one common stencil offset (the lanes of one atomic instruction),
     // spread evenly over the LDS banks (each 8-byte word address mod 32 hit
-    // by two lanes, the fewest possible for 64 lanes; with x-planes of
-    // JT * (JT + 1) words, three)
-    static constexpr int JSY = JT + JPAD, JSX = JPAD ? (ORDER == 2 ? 104 : 136) : JT * JSY, JSC = JTX * JSX;
+    // by two lanes, the fewest possible for 64 lanes; with unpadded x-planes
+    // of JT * JT words, three or four).  Measured on the beam-plasma: 8 rows
+    // of 10 words in x-planes of 104 (order 2) 49.0 ms, x-planes of 88 49.2,
+    // unpadded rows of 8 in planes of 66 (also two lanes per bank pair at one
+    // offset) 54.2, the earlier 9 rows of 10 in planes of 104 50.2.
+    static constexpr int JSY = JT + 2 * JPAD, JSX = JPAD ? (ORDER == 2 ? 104 : 136) : JT * JSY, JSC = JTX * JSX;
     // node accumulation (NACC): per-node staging of the stayers of one
     // 256-particle round, KST entries of 32 bytes per node, and two sets of
     // per-node counters
@@ -535,7 +541,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             k = rel0 >= nk[0] ? 1 : 0;
             rel = rel0 - (k ? nk[0] : 0);
         }
-        const int s = TX == 2 && k ? s0[1] : s0[0];
+        const int s = TX == 2 && k ? s0[TX - 1] : s0[0];
         const int64_t gbase = TX == 2 && k ? cl.gbase[TX - 1] : cl.gbase[0];
         const int jslot = s + rel;
         const int i = perm[jslot];

@@ -195,8 +195,11 @@ k_scan_apply(int32_t *__restrict__ in, int64_t n, const int32_t *__restrict__ bs
 // before they are written back.  Each wavefront takes kScatItems groups of
 // 64 consecutive slots and issues the loads, atomics and stores of the
 // groups together: the kernel is bound by the latency of its dependent
-// accesses (key, count-down, bucket offsets, store), not by bytes.
-constexpr int kScatThreads = 256, kScatItems = 4, kScatTile = kScatThreads * kScatItems;
+// accesses (key, count-down, bucket offsets, store), not by bytes.  Six
+// groups per wavefront measured best on the 1.08e9-particle beam-plasma
+// (sort 5.13 ms per step; four groups 5.40, five 5.26, seven 5.16, eight
+// 5.62).
+constexpr int kScatThreads = 256, kScatItems = 6, kScatTile = kScatThreads * kScatItems;
 
 __global__ void __launch_bounds__(kScatThreads)
 k_scatter(const uint32_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ cell_off,

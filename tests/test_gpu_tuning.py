@@ -43,10 +43,14 @@ def test_autotune_keeps_state(precision):
     a.step(2)
     b.step(2)
     knobs = b.autotune(reps=2)
-    assert knobs["threads"] in (128, 256, 512) and knobs["interleave"] in (0, 1)
+    # layouts: cell-major, lane-interleaved, and (mixed form) pair-interleaved
+    assert knobs["threads"] in (128, 256, 512)
+    assert knobs["interleave"] in ((0, 1, 2) if precision == sa.MIXED else (0, 1))
     assert knobs["persistent"] in (0, 1)
     assert knobs["xcd"] in (0, 1) and not (knobs["xcd"] and knobs["persistent"])
-    assert knobs["tile"] in (0, 1)
+    assert knobs["tile"] in (0, 1) and knobs["tile_x"] in (1, 2) and knobs["node_acc"] in (0, 1)
+    if precision != sa.MIXED:
+        assert knobs["tile_x"] == 1 and knobs["node_acc"] == 0
     assert knobs["particles_ms"] > 0
     a.step(5)
     b.step(5)
