@@ -15,6 +15,12 @@ offsets and momenta).
   and total energy differences <= 1e-5 of the total energy, Gauss drift
   <= 1e-6.  One simulation is held at a time (the FP64 state of 1.08e9
   particles takes 134 GB).
+* Laser-wakefield-like uneven patches: 512x128x128 cells, dx = 0.1, dy = dz
+  = 0.5, 1024 patches of 32x16x16 holding 0 to 4096 electrons (ramp +
+  channel, checkerboard particles per cell), a0 = 2 laser as initial field,
+  500 graph-captured steps; the FP64 form in its autotuned launch shape, the
+  mixed form in the persistent work-list launch: field and kinetic energy
+  differences <= 1e-5 of the total energy, Gauss drift <= 1e-6 in both.
 """
 import math
 
@@ -22,6 +28,7 @@ import numpy as np
 import pytest
 
 import smilei_amd as sa
+from tests.plasma import lwfa
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +92,26 @@ def test_beam_plasma_full_size_mixed_vs_fp64_300_steps():
     print("beam-plasma full size, mixed vs fp64 after 300 steps:", err, "gauss drift", drift_m, drift_f)
     assert err["field"] < 1e-5 and err["kinetic"] < 1e-5 and err["total"] < 1e-5, err
     assert drift_m < 1e-6 and drift_f < 1e-6
+
+
+def test_lwfa_full_size_mixed_vs_fp64_500_steps():
+    f = lwfa(sa.Simulation, sa.PLAIN_FP64, use_graph=True)
+    counts = f.patch_counts(0)
+    assert counts.min() == 0 and counts.max() == 4096  # uneven: empty and full patches
+    f.autotune(reps=2)
+    m = lwfa(sa.Simulation, sa.MIXED, use_graph=True)
+    m.set_tuning(sa.TUNE_PERSISTENT, 1)
+    rho0 = float(np.abs(f.rho()).max())
+    gf0, gm0 = f.gauss(), m.gauss()
+    print("run: lwfa, 500 steps each", flush=True)
+    f.step(500)
+    m.step(500)
+    ef, em = f.energies(), m.energies()
+    err = diff(em, ef)
+    drift_f = float(np.abs(f.gauss() - gf0).max()) / rho0
+    drift_m = float(np.abs(m.gauss() - gm0).max()) / rho0
+    print("lwfa full size, mixed vs fp64 after 500 steps:", err, "gauss drift", drift_m, drift_f)
+    assert err["field"] < 1e-5 and err["kinetic"] < 1e-5 and err["total"] < 1e-5, err
+    assert drift_m < 1e-6 and drift_f < 1e-6
+    f.close()
+    m.close()
