@@ -95,3 +95,52 @@ def test_mixed_first_step_matches_oracle_fields():
         ref = o.get_field(fld)
         err = np.abs(sim.get_field(fld) - ref).max() / np.abs(ref).max()
         assert err < 2e-5, (fld, err)
+
+
+def by_weight(p):
+    """Particles in order of their (distinct) weights: matches the two
+    sides particle by particle whatever their storage order."""
+    idx = np.argsort(p["w"], kind="stable")
+    return {k: v[idx] for k, v in p.items()}
+
+
+def test_mixed_ultrarelativistic_particles_against_oracle():
+    """Electrons at gamma ~ 1000 in random directions, in uniform
+    B_z = B_z_m = 0.5 and E_x = 0.01, at low density (n = 1e-3, so that their
+    own field stays small): the mixed form's FP32 momenta and in-cell
+    offsets against the oracle's FP64 push from the same particles, matched
+    by their weights (distinct, w0 (1 + i 2^-20), exact in FP32).  After 1
+    step the cells agree (all but a rounding-level crossing) and momenta /
+    positions agree within 1e-6 of |u| / of a cell; after 10 steps within
+    1e-5."""
+    shape, patch, dx = (32, 16, 16), (8, 8, 8), (0.2, 0.2, 0.2)
+    dt = cfl_dt(dx)
+    sim = sa.Simulation("3Dcartesian", 2, shape, patch, dx, dt, precision=sa.MIXED)
+    o = O.OracleSim(3, 2, shape, patch, dx, dt)
+    p = thermal(shape, dx, 2, 0.0, 31, density=1e-3)
+    n = len(p["x"])
+    p["w"] = p["w"] * (1.0 + np.arange(n) * 2.0 ** -20)
+    rng = np.random.default_rng(32)
+    v = rng.normal(size=(3, n))
+    v *= 1000.0 / np.sqrt((v * v).sum(0))
+    p["px"], p["py"], p["pz"] = v
+    load(sim, sim.add_species("electron", 1.0, -1.0, n), p)
+    o.set_particles(o.add_species(1.0, -1.0), p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    for name, value in (("Bz", 0.5), ("Bz_m", 0.5), ("Ex", 0.01)):
+        a = np.full(shape, value)
+        sim.set_field(name, a)
+        o.set_field(name, a)
+    L = np.array(shape) * np.array(dx)
+    for steps, tol in ((1, 1e-6), (9, 1e-5)):
+        sim.step(steps)
+        o.step(steps)
+        a, b = by_weight(sim.get_particles(0)), by_weight(o.get_particles(0))
+        np.testing.assert_allclose(a["w"], b["w"], rtol=1e-7)
+        if steps == 1:
+            assert np.mean(a["key"] == b["key"]) > 0.999
+        for d, k in enumerate("xyz"):
+            err = np.abs((a[k] - b[k] + 0.5 * L[d]) % L[d] - 0.5 * L[d]).max() / dx[d]
+            assert err < tol, (steps, k, err)
+        for k in ("px", "py", "pz"):
+            err = np.abs(a[k] - b[k]).max() / 1000.0
+            assert err < tol, (steps, k, err)
