@@ -106,7 +106,9 @@ int smi_destroy(smi_context *ctx);
 int smi_add_species(smi_context *ctx, const char *name, double mass, double charge,
                     int64_t capacity, int *species_id);
 /* Load particles (global coordinates, momenta u = gamma*beta in units of
- * m_species*c, weights in the reference convention n*V_cell/ppc). */
+ * m_species*c, weights in the reference convention n*V_cell/ppc).
+ * SMI_ERR_INVALID, with the species left as it was, for a position outside
+ * [0, grid length) or a non-finite momentum or weight. */
 int smi_set_particles(smi_context *ctx, int species, int64_t n,
                       const double *x, const double *y, const double *z,
                       const double *px, const double *py, const double *pz,

@@ -637,6 +637,12 @@ int smi_set_particles(smi_context *c, int s, int64_t n, const double *x, const d
             if (!(v[i] >= 0. && v[i] < c->g.xmax[d]))
                 throw Error(SMI_ERR_INVALID, "particle position outside the simulation box");
     }
+    // a non-finite momentum would move its particle by a non-finite offset
+    // and spread NaN through J; a non-finite weight also sets the mixed
+    // form's fixed-point unit
+    for (int q = 3; q < 7; ++q)
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(src[q][i])) throw Error(SMI_ERR_INVALID, "non-finite particle momentum or weight");
     SMI_HIP(hipSetDevice(c->p.device));
     drop_graphs(c);
     SMI_HIP(hipStreamSynchronize(c->st));

@@ -53,6 +53,24 @@ def test_capacity_is_enforced():
         sim.add_species("huge", 1.0, -1.0, 1 << 31)
 
 
+@pytest.mark.parametrize("precision", [sa.PLAIN_FP64, sa.MIXED])
+@pytest.mark.parametrize("key,value", [("x", np.nan), ("x", -0.1), ("px", np.nan), ("pz", np.inf), ("w", np.nan)])
+def test_corrupt_particles_are_refused(precision, key, value):
+    """Non-finite or out-of-box input is refused by smi_set_particles
+    (SMI_ERR_INVALID) before anything reaches the device; the species keeps
+    its previous particles."""
+    sim = new(precision)
+    p = thermal(SHAPE, DX, 1, 0.1, 2)
+    s = sim.add_species("e", 1.0, -1.0, len(p["x"]))
+    load(sim, s, p)
+    bad = {k: v.copy() for k, v in p.items()}
+    bad[key][7] = value
+    with pytest.raises(RuntimeError, match="outside the simulation box|non-finite"):
+        load(sim, s, bad)
+    assert sim.num_particles(s) == len(p["x"])
+    sim.step(1)
+
+
 def dense_plasma(seed=5, n_dense=3000, sigma=0.3):
     """A thin background plus n_dense particles in one cell (8, 8, 8), fast
     enough that most of them change node every step."""
