@@ -363,7 +363,7 @@ def main():
         print(json.dumps({"copy_GBs": copy_probe(), "bytes_per_copy": 4 << 30}))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the PIC step is not sharded across GPUs here); "
+        print("bench.py: one GPU only (the library does not shard the PIC step across GPUs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     W = workload(args.workload, args.ppc)

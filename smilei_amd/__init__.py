@@ -1,6 +1,6 @@
 """smilei_amd -- MI355X-native PIC time step of Smilei behind a C ABI.
 
-Python is only the driver: the particle loop, the sort/exchange and the Yee
+Python only orchestrates: the particle loop, the sort/exchange and the Yee
 solver run in libsmilei_amd.so (HIP, gfx950).  ``Simulation`` wraps one
 context of the C ABI (include/smilei_amd.h); ``profiles`` evaluates
 namelist-style density / particles-per-cell profiles and external fields on
