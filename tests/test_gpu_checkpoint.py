@@ -46,3 +46,23 @@ def test_restart_continues_the_run(tmp_path, precision, deposit, tol):
             np.testing.assert_array_equal(fa, fb, err_msg=f)
         else:
             assert np.abs(fa - fb).max() <= tol * max(np.abs(fa).max(), 1e-300), f
+
+
+def test_restarted_run_matches_oracle(tmp_path):
+    """Deterministic FP64 form: 5 steps, a checkpoint, a new simulation
+    restarted from it, 5 more steps; bit-identical (particles, keys, counts,
+    every field) to the oracle run 10 steps from the same initial state."""
+    from oracle import pyoracle as O
+    from tests.test_gpu_configs import compare, copy_to_oracle
+    a = make(sa.PLAIN_FP64, sa.DEPOSIT_DETERMINISTIC)
+    dx = (0.2, 0.2, 0.2)
+    o = O.OracleSim(3, 2, (32, 32, 32), (16, 16, 16), dx, cfl_dt(dx))
+    copy_to_oracle(a, 2, [1.0, 1836.0], [-1.0, 1.0], o)
+    a.step(5)
+    path = str(tmp_path / "ckpt.npz")
+    a.checkpoint(path)
+    a.close()
+    b = sa.Simulation.restart(path)
+    b.step(5)
+    o.step(10)
+    compare(b, o, 2, exact=True)
