@@ -41,6 +41,7 @@ struct Geom {
     int npatches;     // total patches
     int64_t ncells;   // total cells
     double dx[3], dinv[3], dt;
+    double dxdt[3];   // dx/dt (the mixed deposit's current factor)
     double plen[3];   // patch length n*dx, as Patch::initStep3 computes it
     double xmax[3];   // global length dx*N, as Patch::prepareParticles does
 };

@@ -482,6 +482,7 @@ int smi_create(const smi_params *p, smi_context **out)
     g.cells_per_cl = nd == 3 ? 64 : 16;
     g.npatches = g.npatch[0] * g.npatch[1] * g.npatch[2];
     g.dt = p->timestep;
+    for (int d = 0; d < 3; ++d) g.dxdt[d] = g.dx[d] / g.dt;
 
     SMI_HIP(hipSetDevice(p->device));
     auto *c = new smi_context();

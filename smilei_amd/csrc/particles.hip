@@ -116,7 +116,7 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
         const int gi = wrapi(cl.G0[0] - TL::GO + a, g.N[0]);
         const int gj = wrapi(cl.G0[1] - TL::GO + b, g.N[1]);
         const int gk = ND == 3 ? wrapi(cl.G0[2] - TL::GO + c, g.N[2]) : 0;
-        const int64_t idx = ((int64_t)gi * g.N[1] + gj) * g.N[2] + gk;
+        const int idx = (gi * g.N[1] + gj) * g.N[2] + gk;  // fewer than 2^27 cells
         const int so = ND == 3 ? a * GSX + b * GT + c : t;
 #pragma unroll
         for (int f = 0; f < 6; ++f) sF[f * GF + so] = src[f][idx];
@@ -351,9 +351,11 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
         __syncthreads();
     }
     if constexpr (DEPOSIT) {
-        double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
-        for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
-            const int c = t / JT3, r = t % JT3;
+        // one tile position per thread and pass, its grid index computed once
+        // for the three components (32-bit: the engine holds fewer than 2^27
+        // cells)
+        const gptr<double> dstJ[3] = {glb(J.Jx), glb(J.Jy), glb(J.Jz)};
+        for (int r = threadIdx.x; r < JT3; r += blockDim.x) {
             int a, b, k;
             if constexpr (ND == 3) {
                 a = r / (JT * JT);
@@ -364,12 +366,15 @@ __device__ __attribute__((always_inline)) inline void fp_cluster(const Geom &g, 
                 b = r % JT;
                 k = 0;
             }
-            const double v = ND == 3 ? sJ[c * JSC + a * JSX + b * JSY + k] : sJ[t];
-            if (v == 0.) continue;
             const int gi = wrapi(cl.G0[0] - TL::JO + a, g.N[0]);
             const int gj = wrapi(cl.G0[1] - TL::JO + b, g.N[1]);
             const int gk = ND == 3 ? wrapi(cl.G0[2] - TL::JO + k, g.N[2]) : 0;
-            unsafeAtomicAdd(&dstJ[c][((int64_t)gi * g.N[1] + gj) * g.N[2] + gk], v);
+            const int idx = (gi * g.N[1] + gj) * g.N[2] + gk;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double v = ND == 3 ? sJ[c * JSC + a * JSX + b * JSY + k] : sJ[c * JT3 + r];
+                if (v != 0.) unsafeAtomicAdd((double *)(dstJ[c] + idx), v);
+            }
         }
     }
 }

@@ -465,7 +465,8 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     constexpr int GSX = TL::GSX, GSY = TL::GSY, GF = TL::GF;
     constexpr int GT3 = TL::GTX * GT * GT, JT3 = JTX * JT * JT;
     __shared__ float sF[6 * GF];
-    __shared__ unsigned long long sJ[3 * TL::JSC];
+    static_assert(TL::JSC % 2 == 0, "the J tile is cleared 16 bytes at a time");
+    __shared__ __align__(16) unsigned long long sJ[3 * TL::JSC];
     __shared__ int sHist[TX][kMaxMerged][64];
     // particles whose node changes on some axis: their deposit (the wider
     // stencil) is deferred to a second pass over this queue, so that the
@@ -494,12 +495,13 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     const double *src[6] = {F.Ex, F.Ey, F.Ez, F.Bx, F.By, F.Bz};
     for (int t = threadIdx.x; t < GT3; t += blockDim.x) {
         const int a = t / (GT * GT), b = (t / GT) % GT, c = t % GT;
-        const int64_t idx = ((int64_t)wrapi(cl.G0[0] - GO + a, g.N[0]) * g.N[1] + wrapi(cl.G0[1] - GO + b, g.N[1])) *
-                                g.N[2] + wrapi(cl.G0[2] - GO + c, g.N[2]);
+        // 32-bit index: the engine holds fewer than 2^27 cells
+        const int idx = (wrapi(cl.G0[0] - GO + a, g.N[0]) * g.N[1] + wrapi(cl.G0[1] - GO + b, g.N[1])) * g.N[2] +
+                        wrapi(cl.G0[2] - GO + c, g.N[2]);
 #pragma unroll
         for (int f = 0; f < 6; ++f) sF[f * GF + a * GSX + b * GSY + c] = (float)src[f][idx];
     }
-    for (int t = threadIdx.x; t < 3 * TL::JSC; t += blockDim.x) sJ[t] = 0ull;
+    for (int t = threadIdx.x; t < 3 * TL::JSC / 2; t += blockDim.x) reinterpret_cast<ulonglong2 *>(sJ)[t] = ulonglong2{0ull, 0ull};
     for (int t = threadIdx.x; t < TX * kMaxMerged * 64; t += blockDim.x) (&sHist[0][0][0])[t] = 0;
     if constexpr (NACC)
         for (int t = threadIdx.x; t < 2 * NN; t += blockDim.x) (&sCnt[0][0])[t] = 0;
@@ -518,7 +520,7 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
     }
     // cr_c = q w / V * dx_c / dt with one common factor: the ratio between
     // components must be exact for the discrete continuity equation
-    const double dxdt[3] = {g.dx[0] / g.dt, g.dx[1] / g.dt, g.dx[2] / g.dt};
+    const double dxdt[3] = {g.dxdt[0], g.dxdt[1], g.dxdt[2]};
     for (int sid = 0; sid < ns; ++sid) {
     const MixSpecies &S = specs[sid];
     const SpeciesKM &sp = S.k;
@@ -802,16 +804,22 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
         deposit_fixed<ORDER, ORDER + 2, ORDER + 1, JPAD, TX>(sJ, dold, dnew, sh, o3, crq);
     }
     __syncthreads();
-    double *dstJ[3] = {J.Jx, J.Jy, J.Jz};
-    for (int t = threadIdx.x; t < 3 * JT3; t += blockDim.x) {
-        const int c = t / JT3, r = t % JT3;
+    // flush: one tile position per thread and pass, its grid index computed
+    // once for the three components (32-bit: the engine holds fewer than
+    // 2^27 cells)
+    const gptr<double> dstJ[3] = {glb(J.Jx), glb(J.Jy), glb(J.Jz)};
+    for (int r = threadIdx.x; r < JT3; r += blockDim.x) {
         const int a = r / (JT * JT), b = (r / JT) % JT, k = r % JT;  // a < JTX
-        const double v = fixed_decode(sJ[c * TL::JSC + a * TL::JSX + b * TL::JSY + k], quantum);
-        if (v == 0.) continue;
         const int gi = wrapi(cl.G0[0] - JO + a, g.N[0]);
         const int gj = wrapi(cl.G0[1] - JO + b, g.N[1]);
         const int gk = wrapi(cl.G0[2] - JO + k, g.N[2]);
-        unsafeAtomicAdd(&dstJ[c][((int64_t)gi * g.N[1] + gj) * g.N[2] + gk], v);
+        const int idx = (gi * g.N[1] + gj) * g.N[2] + gk;
+        const unsigned long long *w = &sJ[a * TL::JSX + b * TL::JSY + k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double v = fixed_decode(w[c * TL::JSC], quantum);
+            if (v != 0.) unsafeAtomicAdd((double *)(dstJ[c] + idx), v);
+        }
     }
 }
 
