@@ -51,12 +51,12 @@ BYTES_PER_PUSH = {"fp64": 108.0, "mixed": 60.0}
 # measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 4 GiB device
 # copy): profiles/k_particles_beam1B_traffic.md.  Re-measured when the kernels
 # change; None for workloads without a measurement.
-TRAFFIC_PER_PUSH = {"beam": {"mixed": 134.9, "fp64": 156.4}}
+TRAFFIC_PER_PUSH = {"beam": {"mixed": 135.0, "fp64": 157.7}}
 # VALU instructions per push of the mixed particle kernel on the beam-plasma
 # workload (SQ_INSTS_VALU per launch / particles, rocprofv3 --pmc:
 # profiles/k_particles_mixed_beam1B_sq.md), and the chip's VALU issue peak:
 # 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD.
-VALU_INSTR_PER_PUSH = {"beam": {"mixed": 1034.0 / 64.0}}
+VALU_INSTR_PER_PUSH = {"beam": {"mixed": 969.0 / 64.0}}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 # configured run length of each workload (steps)
 RUN_STEPS = {"beam": 300, "thermal": 200, "lwfa": 0}
