@@ -129,6 +129,8 @@ def make_sim(args, W, form):
         sim.set_tuning(sa.TUNE_TILE_X, args.tile_x)
     if args.layout >= 0:
         sim.set_tuning(sa.TUNE_INTERLEAVE, args.layout)
+    if args.tile >= 0 and form == "mixed":
+        sim.set_tuning(sa.TUNE_TILE, args.tile)
     if args.node_acc >= 0 and form == "mixed":
         sim.set_tuning(sa.TUNE_NODE_ACC, args.node_acc)
     ncell = int(np.prod(W["shape"]))
@@ -345,6 +347,8 @@ def main():
                     help="cluster layout of the sort: 0 cell-major, 1 lane-, 2 pair-interleaved (with --no-autotune)")
     ap.add_argument("--tile-x", type=int, default=0,
                     help="mixed form: clusters per workgroup along x, 1 or 2 (with --no-autotune)")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="mixed form: 1 = J-tile rows padded, 0 = unpadded (with --no-autotune)")
     ap.add_argument("--node-acc", type=int, default=-1,
                     help="mixed form, order 2: 1 = stayers deposited node by node, 0 = by particle "
                          "(with --no-autotune)")
