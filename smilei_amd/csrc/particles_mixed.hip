@@ -741,9 +741,11 @@ __device__ inline void mixed_cluster(const Geom &g, const MixSpecies *__restrict
             // the particles that leave their cluster (the sort computes it
             // for the others)
             if (rank) {
-                // (the pair-interleaved layout runs the PAIRK kernel)
+                // (by the storage's layout: the pair-interleaved layout also
+                // reaches the order-4 and persistent kernels, which do not
+                // pre-reduce)
                 const int nc = TX == 2 && k ? nk[TX - 1] : nk[0];
-                rank[jslot] = s + (PAIRK ? cluster_rank_pair(rel, nc) : (lay ? cluster_rank(rel, nc) : rel));
+                rank[jslot] = s + layout_rank(lay, rel, nc);
             }
         }
         keysB[jslot] = (tag << 27) | gkey;

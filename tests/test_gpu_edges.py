@@ -252,3 +252,27 @@ def test_canonical_order_independent_of_layout(shape, patch):
         for k in ("x", "y", "z", "px", "py", "pz"):
             scale = np.abs(b[k]).max()
             assert np.abs(a[k] - b[k]).max() <= tol * scale, k
+
+
+@pytest.mark.parametrize("order,persistent", [(4, 0), (2, 1)])
+def test_canonical_order_pair_layout_without_pre_reduction(order, persistent):
+    """The pair-interleaved layout also reaches the mixed kernels that do not
+    pre-reduce pairs (order 4; the persistent work-list launch), which must
+    still rank the particles leaving their cluster by the pair layout: the
+    canonical in-cell order is the same as with the lane-interleaved
+    layout."""
+    shape, patch, dx = (32, 32, 32), (16, 16, 16), (0.2, 0.2, 0.2)
+    sims = []
+    for lay in (1, 2):
+        sim = sa.Simulation("3Dcartesian", order, shape, patch, dx, cfl_dt(dx), precision=sa.MIXED)
+        sim.set_tuning(sa.TUNE_INTERLEAVE, lay)
+        sim.set_tuning(sa.TUNE_PERSISTENT, persistent)
+        p = thermal(shape, dx, 4, 0.3, 23)
+        load(sim, sim.add_species("e", 1.0, -1.0, len(p["x"])), p)
+        sim.step(6)
+        sims.append(sim)
+    a, b = sims[1].get_particles(0), sims[0].get_particles(0)
+    for k in ("patch", "key"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    for k in ("x", "y", "z", "px", "py", "pz"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-6 * np.abs(b[k]).max(), k
