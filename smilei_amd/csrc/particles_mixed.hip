@@ -1067,6 +1067,11 @@ void launch_particles_mixed(const Geom &g, const MixSpecies *specs, int ns, doub
                 if (tile) SMI_PM(2, 1, 1);
                 else SMI_PM(2, 0, 1);
             }
+        } else if (paired) {
+            if (tile) hipLaunchKernelGGL((k_particles_mixed<4, 1, 1, true>), grid, block, 0, st, g, specs, ns,
+                                         cr_max, F, J, err, xcd ? 1 : 0);
+            else hipLaunchKernelGGL((k_particles_mixed<4, 0, 1, true>), grid, block, 0, st, g, specs, ns, cr_max,
+                                    F, J, err, xcd ? 1 : 0);
         } else {
             if (tile) SMI_PM(4, 1, 1);
             else SMI_PM(4, 0, 1);

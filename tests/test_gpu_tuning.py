@@ -11,9 +11,10 @@ pytestmark = pytest.mark.gpu
 SPECIES = [("electron", 1.0, -1.0, 0.05), ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))]
 
 
-def make(precision, deposit=sa.DEPOSIT_ATOMIC, shape=(32, 32, 32), seed=9):
+def make(precision, deposit=sa.DEPOSIT_ATOMIC, shape=(32, 32, 32), seed=9, order=2):
     dx = (0.2, 0.2, 0.2)
-    sim = sa.Simulation("3Dcartesian", 2, shape, (16, 16, 16), dx, cfl_dt(dx), precision=precision, deposit=deposit)
+    sim = sa.Simulation("3Dcartesian", order, shape, (16, 16, 16), dx, cfl_dt(dx), precision=precision,
+                        deposit=deposit)
     for k, (name, m, q, sig) in enumerate(SPECIES):
         p = thermal(shape, dx, 8, sig, seed + k)
         load(sim, sim.add_species(name, m, q, len(p["x"])), p)
@@ -196,15 +197,15 @@ def test_two_cluster_patch_tile_agrees(threads, xcd, pad):
         assert np.abs(b.get_field(f) - ref).max() <= 1e-5 * np.abs(ref).max(), f
 
 
-@pytest.mark.parametrize("threads,tile_x", [(256, 1), (512, 2)])
-def test_pair_interleaved_layout_pre_reduction_is_exact(threads, tile_x):
+@pytest.mark.parametrize("threads,tile_x,order", [(256, 1, 2), (512, 2, 2), (256, 1, 4)])
+def test_pair_interleaved_layout_pre_reduction_is_exact(threads, tile_x, order):
     """Pair-interleaved clusters (SMI_TUNE_INTERLEAVE 2): the mixed kernel
     adds the tile words of two lanes whose particles share a node before one
     atomic.  The fixed-point words are integers, so the J tiles hold exactly
     the same sums as with the lane-interleaved layout: after one step the
     particles are identical and J differs only by the order of the FP64
     flush (1e-13); over six steps keys and counts stay equal."""
-    a, b = make(sa.MIXED), make(sa.MIXED)
+    a, b = make(sa.MIXED, order=order), make(sa.MIXED, order=order)
     for sim in (a, b):
         sim.set_tuning(sa.TUNE_THREADS, threads)
         sim.set_tuning(sa.TUNE_TILE_X, tile_x)
