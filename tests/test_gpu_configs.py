@@ -122,12 +122,18 @@ def test_lwfa_mixed_vs_fp64():
     assert err["gauss"] < 1e-6, err
 
 
-def test_order4_mixed_vs_fp64_1000_steps():
+@pytest.mark.parametrize("layout", [1, 2])
+def test_order4_mixed_vs_fp64_1000_steps(layout):
+    """Order-4 mixed form against the FP64 form over 1000 steps; layout 2:
+    the pair-interleaved clusters with the pair pre-reduction of the tile
+    atomics (the shape the autotuner picks at order 4)."""
     shape, patch, dx = (48, 48, 48), (16, 16, 16), (0.2, 0.2, 0.2)
     dt = cfl_dt(dx)
     sims = []
     for prec in (sa.PLAIN_FP64, sa.MIXED):
         sim = sa.Simulation("3Dcartesian", 4, shape, patch, dx, dt, precision=prec)
+        if prec == sa.MIXED:
+            sim.set_tuning(sa.TUNE_INTERLEAVE, layout)
         for k, (name, mass, q, sig) in enumerate([("electron", 1.0, -1.0, 0.05),
                                                    ("proton", 1836.0, 1.0, 0.05 / np.sqrt(1836.0))]):
             p = thermal(shape, dx, 8, sig, 40 + k)
